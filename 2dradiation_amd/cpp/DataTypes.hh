@@ -1,0 +1,77 @@
+// Tensor types of the reference API subset used on the stream-collide path
+// (reference: src/DataTypes.hh:37-213, 620-635).  Components are indexed like
+// the reference: Coord/Tensor2 use [1],[2]; Tensor3/Tensor3x3 use [0..2].
+#ifndef RT2D_DATATYPES_HH
+#define RT2D_DATATYPES_HH
+#include "Utility.hh"
+
+struct rank2Indices { int i, j; };
+
+struct Coord {
+    double data[2];
+    Coord(double v = 0) { data[0] = data[1] = v; }
+    Coord(double x, double y) { data[0] = x; data[1] = y; }
+    double &operator[](int k) { return data[k - 1]; }
+    const double &operator[](int k) const { return data[k - 1]; }
+    double EuklNormSquared() const { return data[0] * data[0] + data[1] * data[1]; }
+    double EuklNorm() const { return std::sqrt(EuklNormSquared()); }
+    double Phi() const { return rtm::my_phi(data[0], data[1]); }
+};
+
+struct Tensor2 {
+    double data[2];
+    Tensor2(double v = 0) { data[0] = data[1] = v; }
+    Tensor2(double x, double y) { data[0] = x; data[1] = y; }
+    double &operator[](int k) { return data[k - 1]; }
+    const double &operator[](int k) const { return data[k - 1]; }
+    double EuklNorm() const { return std::sqrt(data[0] * data[0] + data[1] * data[1]); }
+    double Phi() const { return rtm::my_phi(data[0], data[1]); }
+    static double Dot(const Tensor2 &a, const Tensor2 &b) { return a[1] * b[1] + a[2] * b[2]; }
+};
+
+struct Tensor3 {
+    double data[3];
+    Tensor3(double v = 0) { data[0] = data[1] = data[2] = v; }
+    Tensor3(double t, double x, double y) { data[0] = t; data[1] = x; data[2] = y; }
+    double &operator[](int k) { return data[k]; }
+    const double &operator[](int k) const { return data[k]; }
+};
+
+struct Tensor3x3 {
+    double data[9];
+    Tensor3x3(double v = 0) { for (double &x : data) x = v; }
+    Tensor3x3(double tt, double tx, double ty, double xt, double xx, double xy, double yt, double yx,
+              double yy)
+        : data{tt, tx, ty, xt, xx, xy, yt, yx, yy} {}
+    double &operator[](const rank2Indices &k) { return data[k.i * 3 + k.j]; }
+    const double &operator[](const rank2Indices &k) const { return data[k.i * 3 + k.j]; }
+    // 3x3 inverse by the adjugate with cyclic cofactors; the reference calls
+    // Eigen's fixed-size inverse here (src/DataTypes.hh:417-425).
+    Tensor3x3 Invert() const
+    {
+        auto cof = [this](int i, int j) {
+            int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            return data[3 * i1 + j1] * data[3 * i2 + j2] - data[3 * i1 + j2] * data[3 * i2 + j1];
+        };
+        double c0 = cof(0, 0), c1 = cof(1, 0), c2 = cof(2, 0);
+        double det = (c0 * data[0] + c1 * data[3]) + c2 * data[6];
+        double id = 1.0 / det;
+        Tensor3x3 r;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) r.data[3 * i + j] = cof(j, i) * id;
+        return r;
+    }
+};
+
+// RotationMatrix (src/DataTypes.hh:620-635).
+struct RotationMatrix {
+    double angle;
+    explicit RotationMatrix(double a) : angle(a) {}
+    RotationMatrix Inverse() const { return RotationMatrix(std::fmod(-angle + 2.0 * rtm::kPi, 2.0 * rtm::kPi)); }
+};
+inline Tensor2 operator*(const RotationMatrix &m, const Tensor2 &t)
+{
+    double s = MySin(m.angle), c = MyCos(m.angle);
+    return Tensor2(t[1] * c - t[2] * s, t[1] * s + t[2] * c);
+}
+#endif

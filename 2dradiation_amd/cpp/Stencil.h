@@ -1,0 +1,43 @@
+// Velocity stencil and its cubic angular interpolation grid
+// (reference: src/Stencil.h, src/Stencil.cpp:4-162, 177-297).  Ghost
+// directions (adaptive streaming) and the initial-data lookup tables are not
+// part of this library: nGhost must be 0.
+#ifndef RT2D_STENCIL_H
+#define RT2D_STENCIL_H
+#include <array>
+#include "DataTypes.hh"
+
+struct Stencil;
+
+struct InterpolationGrid {
+    size_t nGrid = 0;
+    std::vector<std::array<size_t, 4>> neighbourIndexesCubic;
+    std::vector<std::array<double, 4>> neighbourWeightsCubic;
+
+    InterpolationGrid() = default;
+    InterpolationGrid(size_t nGrid, const Stencil &stencil);
+    double d(double phi) const;
+    double Phi(size_t d) const { return 2.0 * rtm::kPi * d / nGrid; }
+};
+
+struct Stencil {
+    std::string name;
+    size_t nDir, nGhost, nOrder, nCoefficients;
+    InterpolationGrid interpolationGrid;
+
+  private:
+    DoubleBuffer w, phi, cx, cy;
+
+  public:
+    Stencil(size_t nOrder, int nGhost = 0, bool isIntensityStencil = true);
+    double W(size_t d) const { return w[d]; }
+    double Phi(size_t d) const { return phi[d]; }
+    double Cx(size_t d) const { return cx[d]; }
+    double Cy(size_t d) const { return cy[d]; }
+    Tensor2 C(size_t d) const { return Tensor2(cx[d], cy[d]); }
+    const double *WData() const { return w.data(); }
+    const double *PhiData() const { return phi.data(); }
+    const double *CxData() const { return cx.data(); }
+    const double *CyData() const { return cy.data(); }
+};
+#endif

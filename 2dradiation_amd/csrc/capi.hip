@@ -1,0 +1,638 @@
+/*
+ * C ABI of the stream-collide library (declarations and reference mapping in
+ * include/rt2d.h).  Host-side orchestration only; kernels live in
+ * kernels_ref.hip and kernels_fused.hip.
+ */
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "rt_internal.h"
+#include "../cpp/Spacetimes.h"
+#include "../cpp/Stencil.h"
+
+int launch_mask(rt_ctx *c);
+
+namespace {
+thread_local std::string g_create_error;
+
+template <typename P>
+int dalloc(rt_ctx *c, P **p, size_t bytes)
+{
+    if (*p) return RT_OK;
+    hipError_t e = hipMalloc((void **)p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return rt_fail(c, RT_ERR_OOM, std::string("device allocation of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    return RT_OK;
+}
+template <typename P>
+void dfree(P *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+bool all_finite(const double *a, size_t n)
+{
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(a[i])) return false;
+    return true;
+}
+size_t tsize(const rt_ctx *c) { return c->fp32 ? sizeof(float) : sizeof(double); }
+
+#define TRY(x)                                                                                     \
+    do {                                                                                           \
+        int _r = (x);                                                                              \
+        if (_r != RT_OK) return _r;                                                                \
+    } while (0)
+#define NEED(c, p)                                                                                 \
+    do {                                                                                           \
+        if (!(p)) return rt_fail((c), RT_ERR_ARG, "null pointer argument: " #p);                    \
+    } while (0)
+
+/* Builds the per-direction constants of the fused kernel from the stencil and,
+ * for the curved path, checks that the cubic tables repeat with period 10 fine
+ * angles (true for the uniform stencils Stencil(n,0) builds). */
+int build_dir_tables(rt_ctx *c, const double *w, const double *phi, const double *cx, const double *cy,
+                     const int32_t *Nc, const double *Wc)
+{
+    const int D = c->D;
+    c->fused_tables_ok = false;
+    if (D > RT_MAX_FUSED_DIR) return RT_OK;
+    DirTables *h = new DirTables();
+    memset(h, 0, sizeof *h);
+    const rt_desc &d = c->desc;
+    for (int k = 0; k < D; k++) {
+        for (int q = 0; q < c->nCoeff && q < 8; q++) {
+            h->basis[k][q] = rtm::fourier_basis(q, phi[k]);
+            h->basisf[k][q] = (float)h->basis[k][q];
+        }
+        const double X[6] = {1.0, cx[k], cy[k], cx[k] * cx[k], cx[k] * cy[k], cy[k] * cy[k]};
+        for (int q = 0; q < 6; q++) {
+            h->w6[k][q] = w[k] * X[q];
+            h->S[0][q] += w[k] * X[q];
+            h->S[1][q] += w[k] * cx[k] * X[q];
+            h->S[2][q] += w[k] * cy[k] * X[q];
+        }
+        h->cx[k] = cx[k];
+        h->cy[k] = cy[k];
+        h->cxf[k] = (float)cx[k];
+        h->cyf[k] = (float)cy[k];
+        const double di = -cx[k] * d.dt / d.dx, dj = -cy[k] * d.dt / d.dy;
+        const double fi = std::floor(di), fj = std::floor(dj);
+        h->flat_di[k] = (int)fi;
+        h->flat_dj[k] = (int)fj;
+        h->flat_fx[k] = di - fi;
+        h->flat_fy[k] = dj - fj;
+        h->flat_fxf[k] = (float)(di - fi);
+        h->flat_fyf[k] = (float)(dj - fj);
+    }
+    const int nG = c->nGrid;
+    h->nGrid = nG;
+    h->quarter = nG / 4;
+    h->ang_u = nG / (2.0 * rtm::kPi);
+    h->ang_uf = (float)h->ang_u;
+    bool ok = (nG == 10 * D) && (nG % 4 == 0) && Nc && Wc && D >= 16;
+    /* Weights of fine angle g (and of g+1) over directions b-4..b+3 with
+     * b = g/10; a window that differs only by taps of (numerically) zero
+     * weight, as at fine angles that coincide with a stencil direction,
+     * expands to the same vector. */
+    auto expand = [&](int g, int b, double out[8]) {
+        for (int k = 0; k < 8; k++) out[k] = 0;
+        for (int m = 0; m < 4; m++) {
+            int rel = ((Nc[4 * g + m] - b) % D + D) % D;
+            if (rel > D / 2) rel -= D;
+            if (rel < -4 || rel > 3) return false;
+            out[rel + 4] += Wc[4 * g + m];
+        }
+        return true;
+    };
+    for (int r = 0; ok && r < 10; r++) {
+        double W0r[8], W1r[8];
+        ok = expand(r, 0, W0r) && expand((r + 1) % nG, 0, W1r);
+        for (int g = r; ok && g < nG; g += 10) {
+            double W0[8], W1[8];
+            const int b = g / 10;
+            ok = expand(g, b, W0) && expand((g + 1) % nG, b, W1);
+            for (int k = 0; ok && k < 8; k++)
+                if (std::fabs(W0[k] - W0r[k]) > 1e-12 || std::fabs(W1[k] - W1r[k]) > 1e-12) ok = false;
+        }
+        if (!ok) break;
+        int pmin = 8, pmax = -1;
+        for (int k = 0; k < 8; k++)
+            if (std::fabs(W0r[k]) > 1e-13 || std::fabs(W1r[k]) > 1e-13) {
+                pmin = std::min(pmin, k);
+                pmax = std::max(pmax, k);
+            }
+        if (pmax < 0 || pmax - pmin > 4) {
+            ok = false;
+            break;
+        }
+        h->woff[r] = pmin - 4;
+        for (int m = 0; m < 5; m++) {
+            const int k = pmin + m;
+            const double a = k < 8 ? W0r[k] : 0.0, b = k < 8 ? W1r[k] : 0.0;
+            h->wA[r][m] = a;
+            h->wD[r][m] = b - a;
+            h->wAf[r][m] = (float)a;
+            h->wDf[r][m] = (float)(b - a);
+        }
+    }
+    c->fused_tables_ok = ok;
+    hipError_t e = hipMemcpyAsync(c->d_dirt, h, sizeof *h, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    delete h;
+    if (e != hipSuccess) return rt_hip_check(c, e, "upload direction tables");
+    return RT_OK;
+}
+
+int refresh_departure_bound(rt_ctx *c)
+{
+    if (c->desc.streaming != RT_CURVED_FIXED || !c->has_tables || !c->has_stencil) return RT_OK;
+    double m = 0;
+    TRY(launch_departure_bound(c, &m));
+    c->max_disp = m;
+    return RT_OK;
+}
+} // namespace
+
+int rt_fail(rt_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+int rt_hip_check(rt_ctx *c, hipError_t e, const char *what)
+{
+    std::string m = std::string("HIP error in ") + what + ": " + hipGetErrorString(e);
+    return rt_fail(c, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP, m);
+}
+
+extern "C" {
+
+int rt_abi_version(void) { return RT2D_ABI_VERSION; }
+
+const char *rt_last_error(const rt_ctx *c)
+{
+    if (!c) return g_create_error.c_str();
+    return c->err.c_str();
+}
+
+int rt_create(const rt_desc *desc, rt_ctx **out)
+{
+    if (!desc || !out) return rt_fail(nullptr, RT_ERR_ARG, "rt_create: null argument");
+    *out = nullptr;
+    const rt_desc &d = *desc;
+    if (d.nx < 3 || d.ny < 3) return rt_fail(nullptr, RT_ERR_ARG, "Grid must have at least 2 LP in each Dimension (nx, ny >= 3 with halo).");
+    if (!(d.dx > 0 && d.dy > 0 && d.dt > 0) || !std::isfinite(d.dx) || !std::isfinite(d.dy) || !std::isfinite(d.dt))
+        return rt_fail(nullptr, RT_ERR_ARG, "dx, dy, dt must be positive and finite");
+    if (d.nDir < 4 || d.nDir > 4096) return rt_fail(nullptr, RT_ERR_ARG, "nDir must be in [4, 4096]");
+    if (d.nCoeff < 1 || d.nCoeff > 8) return rt_fail(nullptr, RT_ERR_ARG, "nCoeff must be in [1, 8]");
+    if (d.precision != RT_FP64 && d.precision != RT_FP32_MIXED) return rt_fail(nullptr, RT_ERR_ARG, "unknown precision");
+    if (d.streaming == RT_FLAT_ADAPTIVE || d.streaming == RT_CURVED_ADAPTIVE || d.streaming == RT_GEODESIC_FIXED)
+        return rt_fail(nullptr, RT_ERR_UNSUPPORTED, "only FlatFixed and CurvedFixed streaming are implemented");
+    if (d.streaming != RT_FLAT_FIXED && d.streaming != RT_CURVED_FIXED) return rt_fail(nullptr, RT_ERR_ARG, "Invalid StreamingType");
+    if ((long long)d.nx * d.ny * d.nDir > (1LL << 33)) return rt_fail(nullptr, RT_ERR_ARG, "problem too large");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return rt_fail(nullptr, RT_ERR_HIP, "no HIP device available");
+    if (d.device < 0 || d.device >= ndev) return rt_fail(nullptr, RT_ERR_ARG, "device ordinal out of range");
+    e = hipSetDevice(d.device);
+    if (e != hipSuccess) return rt_fail(nullptr, RT_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+
+    rt_ctx *c = new rt_ctx();
+    c->desc = d;
+    c->N = d.nx * d.ny;
+    c->D = d.nDir;
+    c->nCoeff = d.nCoeff;
+    c->fp32 = d.precision == RT_FP32_MIXED;
+    int r = RT_OK;
+    auto fail = [&](int rc) {
+        g_create_error = c->err;
+        rt_destroy(c);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(rt_fail(c, RT_ERR_HIP, "stream creation failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(rt_fail(c, RT_ERR_HIP, "event creation failed"));
+    const size_t N = c->N, D = c->D;
+    if ((r = dalloc(c, &c->d_I[0], N * D * tsize(c))) || (r = dalloc(c, &c->d_I[1], N * D * tsize(c))) ||
+        (r = dalloc(c, &c->d_mom, 6 * N * sizeof(double))) || (r = dalloc(c, &c->d_count, N * sizeof(int32_t))) ||
+        (r = dalloc(c, &c->d_stats, 2 * sizeof(unsigned long long))) || (r = dalloc(c, &c->d_mask, N)) ||
+        (r = dalloc(c, &c->d_f64, 6 * N * sizeof(double))) || (r = dalloc(c, &c->d_cell, RT_CELL_FIELDS * N * tsize(c))) ||
+        (r = dalloc(c, &c->d_alpha, N * sizeof(double))) || (r = dalloc(c, &c->d_t11, N * sizeof(double))) ||
+        (r = dalloc(c, &c->d_t12, N * sizeof(double))) || (r = dalloc(c, &c->d_t21, N * sizeof(double))) ||
+        (r = dalloc(c, &c->d_t22, N * sizeof(double))) || (r = dalloc(c, &c->d_dirt, sizeof(DirTables))) ||
+        (r = dalloc(c, &c->d_w, D * sizeof(double))) || (r = dalloc(c, &c->d_phi, D * sizeof(double))) ||
+        (r = dalloc(c, &c->d_cx, D * sizeof(double))) || (r = dalloc(c, &c->d_cy, D * sizeof(double))) ||
+        (r = dalloc(c, &c->d_basis, D * d.nCoeff * sizeof(double))))
+        return fail(r);
+    if (d.streaming == RT_CURVED_FIXED && (r = dalloc(c, &c->d_coef, 5 * (size_t)d.nCoeff * N * sizeof(double)))) return fail(r);
+    /* zeroed state; flat geometry until rt_set_geometry (alpha = 1, tetrad = I) */
+    if (hipMemsetAsync(c->d_I[0], 0, N * D * tsize(c), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_I[1], 0, N * D * tsize(c), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_mom, 0, 6 * N * sizeof(double), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_stats, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_f64, 0, 6 * N * sizeof(double), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_t12, 0, N * sizeof(double), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_t21, 0, N * sizeof(double), c->stream) != hipSuccess)
+        return fail(rt_fail(c, RT_ERR_HIP, "initialisation failed"));
+    if ((r = launch_fill(c, c->d_alpha, N, 1.0)) || (r = launch_fill(c, c->d_t11, N, 1.0)) || (r = launch_fill(c, c->d_t22, N, 1.0)) ||
+        (r = launch_mask(c)))
+        return fail(r);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(rt_fail(c, RT_ERR_HIP, "initialisation failed"));
+    *out = c;
+    return RT_OK;
+}
+
+void rt_destroy(rt_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dfree(c->d_w); dfree(c->d_phi); dfree(c->d_cx); dfree(c->d_cy); dfree(c->d_Nc); dfree(c->d_Wc);
+    dfree(c->d_basis); dfree(c->d_dirt); dfree(c->d_alpha); dfree(c->d_t11); dfree(c->d_t12);
+    dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_mask); dfree(c->d_f64);
+    dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_mom); dfree(c->d_count);
+    dfree(c->d_stats);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int rt_synchronize(rt_ctx *c)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_set_stencil(rt_ctx *c, const double *w, const double *phi, const double *cx, const double *cy, int32_t nGrid,
+                   const int32_t *Nc, const double *Wc)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    NEED(c, w); NEED(c, phi); NEED(c, cx); NEED(c, cy);
+    const int D = c->D;
+    if (!all_finite(w, D) || !all_finite(phi, D) || !all_finite(cx, D) || !all_finite(cy, D))
+        return rt_fail(c, RT_ERR_NONFINITE, "stencil arrays contain NaN/Inf");
+    if (c->desc.streaming == RT_CURVED_FIXED) {
+        if (nGrid < 2 || !Nc || !Wc) return rt_fail(c, RT_ERR_ARG, "curved streaming needs the cubic interpolation tables");
+        for (long k = 0; k < 4L * nGrid; k++)
+            if (Nc[k] < 0 || Nc[k] >= D) return rt_fail(c, RT_ERR_ARG, "interpolation table index out of range");
+        if (!all_finite(Wc, 4L * nGrid)) return rt_fail(c, RT_ERR_NONFINITE, "interpolation weights contain NaN/Inf");
+    }
+    RT_HIP(c, hipMemcpyAsync(c->d_w, w, D * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(c->d_phi, phi, D * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(c->d_cx, cx, D * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(c->d_cy, cy, D * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<double> basis((size_t)D * c->nCoeff);
+    for (int d = 0; d < D; d++)
+        for (int k = 0; k < c->nCoeff; k++) basis[(size_t)d * c->nCoeff + k] = rtm::fourier_basis(k, phi[d]);
+    RT_HIP(c, hipMemcpyAsync(c->d_basis, basis.data(), basis.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (Nc && Wc && nGrid > 0) {
+        if (c->nGrid != nGrid) {
+            dfree(c->d_Nc);
+            dfree(c->d_Wc);
+        }
+        c->nGrid = nGrid;
+        TRY(dalloc(c, &c->d_Nc, 4 * (size_t)nGrid * sizeof(int32_t)));
+        TRY(dalloc(c, &c->d_Wc, 4 * (size_t)nGrid * sizeof(double)));
+        RT_HIP(c, hipMemcpyAsync(c->d_Nc, Nc, 4 * (size_t)nGrid * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        RT_HIP(c, hipMemcpyAsync(c->d_Wc, Wc, 4 * (size_t)nGrid * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    TRY(build_dir_tables(c, w, phi, cx, cy, Nc, Wc));
+    c->has_stencil = true;
+    TRY(refresh_departure_bound(c));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_set_streaming_stencil(rt_ctx *c, const double *w, const double *phi, const double *cx, const double *cy)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    NEED(c, w); NEED(c, phi); NEED(c, cx); NEED(c, cy);
+    for (int k = 0; k < c->nCoeff; k++) {
+        c->sw[k] = w[k];
+        c->sphi[k] = phi[k];
+        c->scx[k] = cx[k];
+        c->scy[k] = cy[k];
+    }
+    c->has_sstencil = true;
+    return RT_OK;
+}
+
+int rt_set_geometry(rt_ctx *c, const double *alpha, const double *t11, const double *t12, const double *t21, const double *t22)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    NEED(c, alpha); NEED(c, t11); NEED(c, t12); NEED(c, t21); NEED(c, t22);
+    const size_t N = c->N;
+    if (!all_finite(alpha, N)) return rt_fail(c, RT_ERR_NONFINITE, "alpha contains NaN/Inf");
+    for (size_t k = 0; k < N; k++)
+        if (!(alpha[k] > 0)) return rt_fail(c, RT_ERR_ARG, "alpha must be positive");
+    RT_HIP(c, hipMemcpyAsync(c->d_alpha, alpha, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(c->d_t11, t11, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(c->d_t12, t12, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(c->d_t21, t21, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(c->d_t22, t22, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->has_geometry = true;
+    if (c->has_fields) TRY(launch_prepare_cells(c));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_set_fourier_tables(rt_ctx *c, const double *cS, const double *cX, const double *cY, const double *cCx, const double *cCy)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    if (c->desc.streaming != RT_CURVED_FIXED) return rt_fail(c, RT_ERR_STATE, "Fourier tables are only used by CurvedFixed streaming");
+    NEED(c, cS); NEED(c, cX); NEED(c, cY); NEED(c, cCx); NEED(c, cCy);
+    const size_t N = c->N, F = c->nCoeff;
+    const double *src[5] = {cS, cX, cY, cCx, cCy};
+    std::vector<double> soa(5 * F * N);
+    for (int q = 0; q < 5; q++) {
+        if (!all_finite(src[q], N * F)) return rt_fail(c, RT_ERR_NONFINITE, "Fourier table contains NaN/Inf");
+        for (size_t ij = 0; ij < N; ij++)
+            for (size_t k = 0; k < F; k++) soa[(q * F + k) * N + ij] = src[q][ij * F + k];
+    }
+    RT_HIP(c, hipMemcpyAsync(c->d_coef, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->has_tables = true;
+    TRY(refresh_departure_bound(c));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_build_fourier_tables(rt_ctx *c, const double *metric, double tol)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    if (c->desc.streaming != RT_CURVED_FIXED) return rt_fail(c, RT_ERR_STATE, "Fourier tables are only used by CurvedFixed streaming");
+    NEED(c, metric);
+    if (!c->has_sstencil) return rt_fail(c, RT_ERR_STATE, "rt_set_streaming_stencil must come first");
+    if (!(tol > 0) || !std::isfinite(tol)) return rt_fail(c, RT_ERR_ARG, "rk45 tolerance must be positive");
+    const size_t bytes = (size_t)c->N * RT_NMETRIC * sizeof(double);
+    if (!all_finite(metric, (size_t)c->N * RT_NMETRIC)) return rt_fail(c, RT_ERR_NONFINITE, "metric block contains NaN/Inf");
+    double *dm = nullptr;
+    TRY(dalloc(c, &dm, bytes));
+    hipError_t e = hipMemcpyAsync(dm, metric, bytes, hipMemcpyHostToDevice, c->stream);
+    int r = RT_OK;
+    if (e != hipSuccess) r = rt_hip_check(c, e, "metric upload");
+    if (r == RT_OK) r = launch_fill(c, c->d_coef, 5 * (size_t)c->nCoeff * c->N, 0.0);
+    if (r == RT_OK) r = launch_geodesic_tables(c, dm, tol);
+    if (r == RT_OK) {
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) r = rt_hip_check(c, e, "geodesic tables");
+    }
+    dfree(dm);
+    if (r != RT_OK) return r;
+    c->has_tables = true;
+    TRY(refresh_departure_bound(c));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_get_fourier_tables(rt_ctx *c, double *cS, double *cX, double *cY, double *cCx, double *cCy)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    if (!c->has_tables) return rt_fail(c, RT_ERR_STATE, "no Fourier tables");
+    const size_t N = c->N, F = c->nCoeff;
+    std::vector<double> soa(5 * F * N);
+    RT_HIP(c, hipMemcpyAsync(soa.data(), c->d_coef, soa.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    double *dst[5] = {cS, cX, cY, cCx, cCy};
+    for (int q = 0; q < 5; q++) {
+        if (!dst[q]) continue;
+        for (size_t ij = 0; ij < N; ij++)
+            for (size_t k = 0; k < F; k++) dst[q][ij * F + k] = soa[(q * F + k) * N + ij];
+    }
+    return RT_OK;
+}
+
+int rt_set_fields(rt_ctx *c, const double *eta, const double *kappa0, const double *kappa1, const double *kappaA, const double *ux,
+                  const double *uy)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    const double *src[6] = {eta, kappa0, kappa1, kappaA, ux, uy};
+    const size_t N = c->N;
+    bool any_u = false;
+    for (int q = 0; q < 6; q++) {
+        if (!src[q]) return rt_fail(c, RT_ERR_ARG, "null field pointer");
+        if (!all_finite(src[q], N)) return rt_fail(c, RT_ERR_NONFINITE, "field contains NaN/Inf");
+    }
+    for (size_t k = 0; k < N; k++) {
+        if (ux[k] != 0.0 || uy[k] != 0.0) any_u = true;
+        if (ux[k] * ux[k] + uy[k] * uy[k] >= 1.0) return rt_fail(c, RT_ERR_ARG, "fluid velocity must be below the speed of light");
+    }
+    for (int q = 0; q < 6; q++)
+        RT_HIP(c, hipMemcpyAsync(c->d_f64 + q * N, src[q], N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->any_u = any_u;
+    c->has_fields = true;
+    TRY(launch_prepare_cells(c));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_upload_I(rt_ctx *c, const double *I)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    NEED(c, I);
+    const size_t n = (size_t)c->N * c->D;
+    if (!all_finite(I, n)) return rt_fail(c, RT_ERR_NONFINITE, "intensities contain NaN/Inf");
+    double *tmp = nullptr;
+    TRY(dalloc(c, &tmp, n * sizeof(double)));
+    int r = RT_OK;
+    hipError_t e = hipMemcpyAsync(tmp, I, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) r = rt_hip_check(c, e, "upload I");
+    if (r == RT_OK) r = launch_transpose_in(c, tmp, c->d_I[c->cur]);
+    if (r == RT_OK) {
+        e = hipMemcpyAsync(c->d_I[1 - c->cur], c->d_I[c->cur], n * tsize(c), hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess) r = rt_hip_check(c, e, "copy I");
+    }
+    if (r == RT_OK) {
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) r = rt_hip_check(c, e, "upload I");
+    }
+    dfree(tmp);
+    if (r == RT_OK) c->has_I = true;
+    return r;
+}
+
+int rt_download_I(rt_ctx *c, double *I)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    NEED(c, I);
+    const size_t n = (size_t)c->N * c->D;
+    double *tmp = nullptr;
+    TRY(dalloc(c, &tmp, n * sizeof(double)));
+    int r = launch_transpose_out(c, c->d_I[c->cur], tmp);
+    hipError_t e;
+    if (r == RT_OK) {
+        e = hipMemcpyAsync(I, tmp, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess) r = rt_hip_check(c, e, "download I");
+    }
+    if (r == RT_OK) {
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) r = rt_hip_check(c, e, "download I");
+    }
+    dfree(tmp);
+    return r;
+}
+
+int rt_get_moments(rt_ctx *c, double *E, double *Fx, double *Fy, double *Pxx, double *Pxy, double *Pyy)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    double *dst[6] = {E, Fx, Fy, Pxx, Pxy, Pyy};
+    for (int q = 0; q < 6; q++)
+        if (dst[q]) RT_HIP(c, hipMemcpyAsync(dst[q], c->d_mom + (size_t)q * c->N, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_iteration_stats(rt_ctx *c, int32_t *maxCount, double *sumCount)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    unsigned long long s[2];
+    RT_HIP(c, hipMemcpyAsync(s, c->d_stats, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    if (maxCount) *maxCount = (int32_t)s[1];
+    if (sumCount) *sumCount = (double)s[0];
+    return RT_OK;
+}
+
+int rt_reset_iteration_stats(rt_ctx *c)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    RT_HIP(c, hipMemsetAsync(c->d_stats, 0, 2 * sizeof(unsigned long long), c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+static int check_ready(rt_ctx *c)
+{
+    if (!c->has_stencil) return rt_fail(c, RT_ERR_STATE, "rt_set_stencil has not been called");
+    if (!c->has_fields) return rt_fail(c, RT_ERR_STATE, "rt_set_fields has not been called");
+    if (c->desc.streaming == RT_CURVED_FIXED) {
+        if (!c->has_tables) return rt_fail(c, RT_ERR_STATE, "curved streaming needs Fourier tables");
+        if (!c->d_Nc) return rt_fail(c, RT_ERR_STATE, "curved streaming needs the cubic interpolation tables");
+    }
+    return RT_OK;
+}
+
+int rt_uses_fused_path(rt_ctx *c, int32_t *fused)
+{
+    if (!c || !fused) return rt_fail(c, RT_ERR_ARG, "null argument");
+    *fused = fused_supported(c) ? 1 : 0;
+    return RT_OK;
+}
+
+int rt_step(rt_ctx *c, int32_t nSteps, int32_t want_moments)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    if (nSteps < 0) return rt_fail(c, RT_ERR_ARG, "nSteps must be >= 0");
+    TRY(check_ready(c));
+    if (nSteps == 0) return RT_OK;
+    if (fused_supported(c)) {
+        /* halo and excised cells are the same in both buffers from here on */
+        TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+        if (want_moments) TRY(launch_fill(c, c->d_mom, 6 * (size_t)c->N, 0.0));
+        for (int n = 0; n < nSteps; n++) {
+            TRY(launch_fused_step(c, c->d_I[c->cur], c->d_I[1 - c->cur], want_moments && n == nSteps - 1));
+            c->cur = 1 - c->cur;
+        }
+    } else {
+        for (int n = 0; n < nSteps; n++) {
+            TRY(launch_ref_stream(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+            c->cur = 1 - c->cur;
+            TRY(launch_ref_moments(c, c->d_I[c->cur]));
+            TRY(launch_ref_collide(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+            c->cur = 1 - c->cur;
+        }
+    }
+    RT_HIP(c, hipGetLastError());
+    return RT_OK;
+}
+
+int rt_stream(rt_ctx *c)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    TRY(check_ready(c));
+    TRY(launch_ref_stream(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+    c->cur = 1 - c->cur;
+    return RT_OK;
+}
+int rt_compute_moments(rt_ctx *c)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    TRY(check_ready(c));
+    return launch_ref_moments(c, c->d_I[c->cur]);
+}
+int rt_collide(rt_ctx *c)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    TRY(check_ready(c));
+    TRY(launch_ref_collide(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+    c->cur = 1 - c->cur;
+    return RT_OK;
+}
+
+int rt_time_steps(rt_ctx *c, int32_t nSteps, float *ms)
+{
+    if (!c || !ms) return rt_fail(c, RT_ERR_ARG, "null argument");
+    RT_HIP(c, hipEventRecord(c->ev0, c->stream));
+    TRY(rt_step(c, nSteps, 0));
+    RT_HIP(c, hipEventRecord(c->ev1, c->stream));
+    RT_HIP(c, hipEventSynchronize(c->ev1));
+    RT_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+    return RT_OK;
+}
+
+int rt_build_stencil(int32_t nOrder, double *w, double *phi, double *cx, double *cy, int32_t *Nc, double *Wc)
+{
+    if (nOrder < 1 || nOrder > 4096 || !w || !phi || !cx || !cy) return rt_fail(nullptr, RT_ERR_ARG, "rt_build_stencil: bad arguments");
+    if ((Nc || Wc) && nOrder < 4) return rt_fail(nullptr, RT_ERR_ARG, "rt_build_stencil: interpolation tables need nOrder >= 4");
+    Stencil s(nOrder, 0, Nc || Wc);
+    for (int d = 0; d < nOrder; d++) {
+        w[d] = s.W(d);
+        phi[d] = s.Phi(d);
+        cx[d] = s.Cx(d);
+        cy[d] = s.Cy(d);
+    }
+    if (Nc && Wc) {
+        const InterpolationGrid &g = s.interpolationGrid;
+        for (size_t i = 0; i < g.nGrid; i++)
+            for (int m = 0; m < 4; m++) {
+                Nc[4 * i + m] = (int32_t)g.neighbourIndexesCubic[i][m];
+                Wc[4 * i + m] = g.neighbourWeightsCubic[i][m];
+            }
+    }
+    return RT_OK;
+}
+
+int rt_build_metric(int32_t kind, double m, double a, int32_t nx, int32_t ny, double dx, double dy, double startx, double starty,
+                    double *metric, double *alpha, double *t11, double *t12, double *t21, double *t22)
+{
+    if (nx < 3 || ny < 3 || !(dx > 0) || !(dy > 0)) return rt_fail(nullptr, RT_ERR_ARG, "rt_build_metric: bad grid");
+    /* rebuild the reference Grid from the stored-grid description */
+    Grid grid(nx - 2, ny - 2, Coord(startx + dx, starty + dy), Coord(startx + (nx - 2) * dx, starty + (ny - 2) * dy));
+    grid.dx = dx;
+    grid.dy = dy;
+    grid.startx = startx;
+    grid.starty = starty;
+    grid.endx = startx + (nx - 1) * dx;
+    grid.endy = starty + (ny - 1) * dy;
+    Metric *M = nullptr;
+    if (kind == 0) M = new Minkowski(grid, m, a);
+    else if (kind == 1) M = new SchwarzSchild(grid, m, a);
+    else if (kind == 2) M = new KerrSchild(grid, m, a);
+    else return rt_fail(nullptr, RT_ERR_ARG, "rt_build_metric: unknown spacetime kind");
+    if (metric) M->PackGeodesicBlock(metric);
+    const size_t N = grid.nxy;
+    for (size_t ij = 0; ij < N; ij++) {
+        if (alpha) alpha[ij] = M->alpha[ij];
+        if (t11) t11[ij] = M->tetrad11_ul[ij];
+        if (t12) t12[ij] = M->tetrad12_ul[ij];
+        if (t21) t21[ij] = M->tetrad21_ul[ij];
+        if (t22) t22[ij] = M->tetrad22_ul[ij];
+    }
+    delete M;
+    return RT_OK;
+}
+
+} /* extern "C" */

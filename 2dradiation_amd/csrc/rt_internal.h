@@ -1,0 +1,105 @@
+/*
+ * Internal declarations shared by the gfx950 kernels and the C-ABI layer.
+ */
+#ifndef RT_INTERNAL_H
+#define RT_INTERNAL_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "../../include/rt2d.h"
+#include "rt_math.h"
+
+#define RT_MAX_FUSED_DIR 64
+#define RT_FUSED_STRIP 64 /* cells per wavefront row segment */
+#define RT_FUSED_WAVES 4
+
+/* Per-direction constants of the fused kernel, uploaded once per stencil.
+ * Indexing by a wave-uniform direction lets the compiler use scalar loads. */
+struct DirTables {
+    double basis[RT_MAX_FUSED_DIR][8];  /* Fourier basis B_k(phi_d), k < nCoeff (<= 8) */
+    float basisf[RT_MAX_FUSED_DIR][8];
+    double w6[RT_MAX_FUSED_DIR][6];     /* w_d * {1, nx, ny, nx^2, nx ny, ny^2} */
+    double cx[RT_MAX_FUSED_DIR], cy[RT_MAX_FUSED_DIR];
+    float cxf[RT_MAX_FUSED_DIR], cyf[RT_MAX_FUSED_DIR];
+    int flat_di[RT_MAX_FUSED_DIR], flat_dj[RT_MAX_FUSED_DIR]; /* floor of the flat departure offset */
+    double flat_fx[RT_MAX_FUSED_DIR], flat_fy[RT_MAX_FUSED_DIR];
+    float flat_fxf[RT_MAX_FUSED_DIR], flat_fyf[RT_MAX_FUSED_DIR];
+    double S[3][6];      /* sum_d w X, sum_d w nx X, sum_d w ny X for X in the moment list */
+    int woff[10];        /* window start offset per fine-angle residue */
+    double wA[10][5], wD[10][5];   /* combined 5-tap angular weights: C = A + t*D */
+    float wAf[10][5], wDf[10][5];
+    double ang_u;        /* nGrid / (2 pi) */
+    float ang_uf;
+    int quarter;         /* nGrid / 4 */
+    int nGrid;
+};
+
+/* Per-cell inputs of the fused kernel, SoA [RT_CELL_FIELDS][N] in the kernel
+ * precision. */
+enum { CF_ETA = 0, CF_K0, CF_K1, CF_KA, CF_UX, CF_UY, CF_ADT, CF_IA3, RT_CELL_FIELDS };
+/* Cell mask bits. */
+enum { MASK_ACTIVE = 1, MASK_NEAR_BH = 2 };
+
+struct rt_ctx {
+    rt_desc desc;
+    int N = 0, D = 0, nCoeff = 0;
+    bool fp32 = false;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    /* stencil */
+    bool has_stencil = false, has_sstencil = false;
+    double *d_w = nullptr, *d_phi = nullptr, *d_cx = nullptr, *d_cy = nullptr;
+    int nGrid = 0;
+    int32_t *d_Nc = nullptr;
+    double *d_Wc = nullptr;
+    double *d_basis = nullptr;    /* [D][nCoeff] Fourier basis at intensity angles (FP64) */
+    double sw[16], sphi[16], scx[16], scy[16]; /* streaming stencil (host copy) */
+    DirTables *d_dirt = nullptr;
+    bool fused_tables_ok = false; /* periodic angular tables verified */
+
+    /* geometry */
+    bool has_geometry = false, has_tables = false;
+    double *d_alpha = nullptr, *d_t11 = nullptr, *d_t12 = nullptr, *d_t21 = nullptr, *d_t22 = nullptr;
+    double *d_coef = nullptr;     /* SoA [5 series][nCoeff][N], FP64 */
+    uint8_t *d_mask = nullptr;
+    double max_disp = 0;          /* max |departure - cell| in cells (curved tables) */
+
+    /* fields */
+    bool has_fields = false;
+    double *d_f64 = nullptr;      /* [6][N] eta, kappa0, kappa1, kappaA, ux, uy */
+    void *d_cell = nullptr;       /* [RT_CELL_FIELDS][N] in kernel precision */
+    bool any_u = false;
+
+    /* state */
+    bool has_I = false;
+    void *d_I[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *d_mom = nullptr;      /* [6][N] */
+    int32_t *d_count = nullptr;   /* [N] per-method collide cycle counts */
+    unsigned long long *d_stats = nullptr; /* [2] sum, max */
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+int rt_fail(rt_ctx *c, int code, const std::string &msg);
+int rt_hip_check(rt_ctx *c, hipError_t e, const char *what);
+#define RT_HIP(c, call)                                                                    \
+    do {                                                                                   \
+        hipError_t _e = (call);                                                            \
+        if (_e != hipSuccess) return rt_hip_check((c), _e, #call);                         \
+    } while (0)
+
+/* kernel launchers (kernels_*.hip) */
+int launch_transpose_in(rt_ctx *c, const double *aos, void *soa);
+int launch_transpose_out(rt_ctx *c, const void *soa, double *aos);
+int launch_copy_fixed_cells(rt_ctx *c, const void *src, void *dst);
+int launch_prepare_cells(rt_ctx *c);
+int launch_ref_stream(rt_ctx *c, const void *in, void *out);
+int launch_ref_moments(rt_ctx *c, const void *in);
+int launch_ref_collide(rt_ctx *c, const void *in, void *out);
+int launch_fused_step(rt_ctx *c, const void *in, void *out, bool want_moments);
+bool fused_supported(const rt_ctx *c);
+int launch_geodesic_tables(rt_ctx *c, const double *d_metric, double tol);
+int launch_departure_bound(rt_ctx *c, double *max_disp);
+int launch_fill(rt_ctx *c, double *p, size_t n, double v);
+#endif

@@ -1,0 +1,178 @@
+"""Throughput benchmark of the fused stream-collide step on one MI355X.
+
+Workload: schwarzschild_1024sq_d64 (1024x1024 cells on [-8,8]^2, 64-direction
+stencil, Schwarzschild M=1 with excision r<=2.1, geodesic tables by RK45,
+eta=max(0,GRF), kappa0~U[0,2], kappa1=0.3 U[0,1], CFL 0.5), seeded synthetic
+fields.  A step is one timestep (stream + moments + collide) over the grid.
+
+Prints one JSON line: FP32-mixed cell*direction updates/s as the value, the
+FP64 GPU path, their ratio and the FP32-vs-FP64 rel-L2 of I, E, Fx, Fy after
+the timed steps, the HBM roofline of the fused kernel and the CPU baseline
+(the oracle, OpenMP on the host cores, on a bounded number of steps).
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOAD = "schwarzschild_1024sq_d64"
+HBM_PEAK = 8.0e12
+
+
+def algorithmic_bytes_per_update(nDir, b_i, b_f, curved=True, nCoeff=5):
+    """2 b_I (read + write I) + (6 fields * b_f + geometry) / nDir, geometry =
+    alpha + 5 series * nCoeff coefficients in FP64 for curved spacetimes."""
+    b_geo = 8 * (1 + 5 * nCoeff) if curved else 0
+    return 2 * b_i + (6 * b_f + b_geo) / nDir
+
+
+def rel_l2(a, b):
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def cpu_baseline(rt, name, budget_s=12.0, max_steps=40):
+    """The oracle's -O3 -ffast-math OpenMP build on the host cores, same
+    workload, as many steps as fit in the time budget (at least 2)."""
+    from oracle import oracle as O
+    spec = rt.workloads.WORKLOADS[name]
+    nx_, ny_, start, end = spec["grid"]
+    og = O.make_grid(nx_, ny_, start, end, spec["cfl"])
+    gs = rt.GridSpec(nx_, ny_, start, end, spec["cfl"])
+    kind = rt.workloads.spacetime_kind(spec)
+    r2 = rt.workloads.excision_r2(spec)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    L = O.lib(fast=True)
+    L.or_set_threads(threads)
+    M = O.metric(og, kind)
+    tabs = O.geodesic_tables(og, M, r2, spec.get("tol", 1e-4)) if kind else None
+    f = rt.workloads.fields(name, gs)
+    P = O.Problem(og, spec["nDir"], 2 if kind else 0, M, bh_r2=r2, tables=tabs, fields=f, fast=True)
+    I = np.zeros(og.N * spec["nDir"])
+    t0 = time.perf_counter()
+    I = P.run(I, 1)[0]
+    t1 = time.perf_counter()
+    per = t1 - t0
+    n = int(max(1, min(max_steps - 1, budget_s / max(per, 1e-6))))
+    t2 = time.perf_counter()
+    P.run(I, n)
+    t3 = time.perf_counter()
+    steps = n
+    elapsed = t3 - t2
+    act = int(np.count_nonzero(_active(gs, r2)))
+    return {"value": act * spec["nDir"] * steps / elapsed, "unit": "cell*dir updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{name}, {steps} timed steps after 1 warm-up step, oracle -O3 -ffast-math OpenMP "
+                      f"({threads} threads), {elapsed:.1f} s"}
+
+
+def _active(gs, r2):
+    x, y = gs.coords()
+    i = np.arange(gs.N) % gs.nx
+    j = np.arange(gs.N) // gs.nx
+    inter = (i > 0) & (i < gs.nx - 1) & (j > 0) & (j < gs.ny - 1)
+    return inter & ~((r2 >= 0) & (x * x + y * y <= r2))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(the domain-decomposition design is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    rt = importlib.import_module("2dradiation_amd")
+    rt.lib()
+    spec = rt.workloads.WORKLOADS[WORKLOAD]
+    K, W = args.steps, args.warmup
+
+    print(f"[bench] setting up {WORKLOAD} FP32-mixed", file=sys.stderr, flush=True)
+    c32, meta = rt.workloads.make_context(WORKLOAD, rt.FP32_MIXED)
+    gs = meta["grid"]
+    act = int(np.count_nonzero(_active(gs, meta["excision_r2"])))
+    upd_per_step = act * spec["nDir"]
+    assert c32.uses_fused(), "fused path not selected"
+    zero = np.zeros(gs.N * spec["nDir"])
+
+    c32.step(W, want_moments=False)
+    c32.synchronize()
+    c32.upload_I(zero)
+    c32.reset_iteration_stats()
+    c32.synchronize()
+    print(f"[bench] timing {K} FP32-mixed steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ms32 = c32.time_steps(K)
+    wall32 = time.perf_counter() - t0
+    rate32 = upd_per_step * K / (ms32 * 1e-3)
+    it_max, it_sum = c32.iteration_stats()
+
+    result = {
+        "metric": "cell*dir updates/s FP32-mixed vs FP64 at rel-L2<=1e-5; adjoint/forward ratio",
+        "value": rate32, "unit": "cell*dir updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": ms32 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 intensities, fp64 tables and moment sums", "data": "synthetic (seeded fields)",
+        "config": {"workload": WORKLOAD, "grid": "1026x1026 stored (1024x1024 interior) on [-8,8]^2",
+                   "nDir": spec["nDir"], "spacetime": "Schwarzschild M=1, excision r<=2.1",
+                   "cfl": spec["cfl"], "steps_per_run": K, "active_cells": act, "parallelism": "single GPU"},
+        "lambda_iterations": {"max": it_max, "mean": it_sum / max(1, act * K)},
+        "wall_s_fp32": wall32,
+    }
+
+    B32 = algorithmic_bytes_per_update(spec["nDir"], 4, 4)
+    per_launch_ms = ms32 / K
+    achieved = B32 * upd_per_step / (per_launch_ms * 1e-3)
+    result["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK, "traffic": None,
+                          "bytes_per_update": B32, "kernel": "k_fused<float,64,curved>"}
+
+    if not args.no_fp64:
+        print(f"[bench] FP64 leg: {K} steps", file=sys.stderr, flush=True)
+        c64, _ = rt.workloads.make_context(WORKLOAD, rt.FP64)
+        c64.step(W, want_moments=False)
+        c64.upload_I(zero)
+        c64.synchronize()
+        ms64 = c64.time_steps(K)
+        rate64 = upd_per_step * K / (ms64 * 1e-3)
+        # quality: both runs started from I=0 and took K steps
+        c32.step(0)
+        I32, I64 = c32.download_I(), c64.download_I()
+        # moments of the final state (one more step with moments requested on copies is not
+        # needed: compare the state's moments computed on the host from I)
+        w, phi, cx, cy = rt.build_stencil(spec["nDir"], tables=False)
+        D = spec["nDir"]
+
+        def mom(I):
+            I2 = I.reshape(-1, D)
+            return I2 @ w, I2 @ (w * cx), I2 @ (w * cy)
+
+        E32, Fx32, Fy32 = mom(I32)
+        E64, Fx64, Fy64 = mom(I64)
+        B64 = algorithmic_bytes_per_update(spec["nDir"], 8, 8)
+        result["fp64"] = {"value": rate64, "ms_per_step": ms64 / K,
+                          "hbm_frac": B64 * upd_per_step / (ms64 / K * 1e-3) / HBM_PEAK}
+        result["fp32_over_fp64"] = rate32 / rate64
+        result["rel_l2_fp32_vs_fp64"] = {"I": rel_l2(I32, I64), "E": rel_l2(E32, E64),
+                                         "Fx": rel_l2(Fx32, Fx64), "Fy": rel_l2(Fy32, Fy64)}
+        c64.close()
+    c32.close()
+
+    if not args.no_cpu:
+        print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(rt, WORKLOAD)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,188 @@
+/*
+ * rt2d.h - C ABI of the MI355X (gfx950) stream-collide library for
+ * 2dRadiation-style general-relativistic lattice-Boltzmann radiation transport.
+ *
+ * The reference (Tom-Olsen/2dRadiation) is a C++/OpenMP code whose hot path is
+ * the per-step loop of Radiation::RunSimulation (src/Radiation.cpp:784-832):
+ * StreamFlatFixed / StreamCurvedFixed, then ComputeMomentsIF, then Collide, with
+ * the geodesic Fourier tables built once by UpdateFourierCoefficients.  It has
+ * no FFI of its own; these entry points sit *under* its C++ methods (the C++
+ * classes in 2dradiation_amd/cpp call them) and are what a ctypes/cgo/JNI
+ * binding would call.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Every entry point returns RT_OK (0) or a negative RT_ERR_* code; the
+ *     message is available from rt_last_error(ctx) (ctx may be NULL for
+ *     errors raised before a context exists).  The reference reports errors by
+ *     ExitOnError (src/Utility.hh:16-20); the C++ layer maps non-zero codes to
+ *     that behaviour.
+ *   - Host arrays use the reference layouts: per-cell arrays [j*nx+i] over the
+ *     stored grid including the 1-cell halo (src/Grid.cpp:44-47); intensities
+ *     AoS I[ij*nDir+d] (src/Radiation.cpp:89-96); Fourier coefficients
+ *     [ij*nCoeff+k] (src/Radiation.cpp:98-101).  Inputs are copied; the caller
+ *     keeps ownership.  On the device intensities are SoA [d][ij].
+ *   - One context is used by one host thread at a time; its work is ordered on
+ *     its own HIP stream.  Separate contexts are independent.
+ */
+#ifndef RT2D_H
+#define RT2D_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT2D_ABI_VERSION 1
+
+enum rt_status {
+    RT_OK = 0,
+    RT_ERR_ARG = -1,         /* invalid argument (sizes, null pointers, ranges) */
+    RT_ERR_STATE = -2,       /* call out of order (e.g. step before fields are set) */
+    RT_ERR_HIP = -3,         /* HIP runtime error */
+    RT_ERR_OOM = -4,         /* device allocation failed */
+    RT_ERR_UNSUPPORTED = -5, /* valid request this build does not implement */
+    RT_ERR_NONFINITE = -6    /* NaN/Inf found in an input field */
+};
+
+enum rt_precision { RT_FP64 = 0, RT_FP32_MIXED = 1 };
+
+/* Values mirror the reference's StreamingType (src/Config.hh:6-13). */
+enum rt_streaming {
+    RT_FLAT_FIXED = 0,
+    RT_FLAT_ADAPTIVE = 1,   /* not implemented: RT_ERR_UNSUPPORTED */
+    RT_CURVED_FIXED = 2,
+    RT_CURVED_ADAPTIVE = 3, /* not implemented: RT_ERR_UNSUPPORTED */
+    RT_GEODESIC_FIXED = 4   /* not implemented: RT_ERR_UNSUPPORTED */
+};
+
+typedef struct rt_ctx rt_ctx;
+
+/* Problem description.  Grid fields are those of the reference Grid
+ * (src/Grid.cpp:4-37): nx, ny count stored cells including the halo. */
+typedef struct rt_desc {
+    int32_t nx, ny;
+    double dx, dy, dt;
+    double startx, starty;
+    int32_t nDir;        /* intensity stencil size (Stencil::nDir) */
+    int32_t nCoeff;      /* Fourier coefficients per series (streaming Stencil::nCoefficients) */
+    int32_t precision;   /* rt_precision */
+    int32_t streaming;   /* rt_streaming */
+    double excision_r2;  /* Metric::InsideBH(xy) is x*x+y*y <= excision_r2; negative: never */
+    int32_t device;      /* HIP device ordinal */
+} rt_desc;
+
+/* Number of metric arrays rt_build_fourier_tables takes, and their order in
+ * the per-cell block metric[ij*RT_NMETRIC + q] (names as in src/Metric.h). */
+#define RT_NMETRIC 29
+enum rt_metric_index {
+    RT_M_ALPHA = 0, RT_M_BETA1_U, RT_M_BETA2_U,
+    RT_M_D1_ALPHA, RT_M_D2_ALPHA,
+    RT_M_D1_BETA1_U, RT_M_D1_BETA2_U, RT_M_D2_BETA1_U, RT_M_D2_BETA2_U,
+    RT_M_K11, RT_M_K12, RT_M_K22,
+    RT_M_GAMMA11_UU, RT_M_GAMMA12_UU, RT_M_GAMMA22_UU,
+    RT_M_D1_GAMMA11_LL, RT_M_D1_GAMMA12_LL, RT_M_D1_GAMMA22_LL,
+    RT_M_D2_GAMMA11_LL, RT_M_D2_GAMMA12_LL, RT_M_D2_GAMMA22_LL,
+    RT_M_TETRAD11, RT_M_TETRAD12, RT_M_TETRAD21, RT_M_TETRAD22,
+    RT_M_TETRADINV11, RT_M_TETRADINV12, RT_M_TETRADINV21, RT_M_TETRADINV22
+};
+
+/* ---- lifetime / diagnostics ------------------------------------------ */
+/* Replaces: construction of Radiation (src/Radiation.cpp:3-83); allocates
+ * the device state for one problem. */
+int rt_create(const rt_desc *desc, rt_ctx **out);
+/* Replaces: Radiation::~Radiation (src/Radiation.cpp:84-87). */
+void rt_destroy(rt_ctx *ctx);
+/* Error text of the last failing call on ctx (or of rt_create if ctx is NULL). */
+const char *rt_last_error(const rt_ctx *ctx);
+/* Waits for all work queued on the context's stream. */
+int rt_synchronize(rt_ctx *ctx);
+int rt_abi_version(void);
+
+/* ---- set-up ------------------------------------------------------------ */
+/* Intensity stencil and its cubic angular interpolation tables.
+ * Replaces: Stencil::{W,Phi,Cx,Cy} and InterpolationGrid::neighbourIndexesCubic /
+ * neighbourWeightsCubic (src/Stencil.h:10-91, src/Stencil.cpp:4-162).
+ * Nc, Wc have nGrid*4 entries. */
+int rt_set_stencil(rt_ctx *ctx, const double *w, const double *phi, const double *cx,
+                   const double *cy, int32_t nGrid, const int32_t *Nc, const double *Wc);
+/* Streaming stencil used for the Fourier fit of the geodesic tables
+ * (Stencil(nF,0,false), exe/main.cpp:547): nCoeff entries each. */
+int rt_set_streaming_stencil(rt_ctx *ctx, const double *w, const double *phi,
+                             const double *cx, const double *cy);
+/* Cell-centre lapse and spatial tetrad (Metric::alpha, tetrad{11,12,21,22}_ul,
+ * src/Metric.h:38, 60-62), each [N]. */
+int rt_set_geometry(rt_ctx *ctx, const double *alpha, const double *t11, const double *t12,
+                    const double *t21, const double *t22);
+/* Geodesic Fourier coefficients computed elsewhere, each [N*nCoeff]
+ * (Radiation::coefficients{S,X,Y,Cx,Cy}, src/Radiation.h:75-79). */
+int rt_set_fourier_tables(rt_ctx *ctx, const double *cS, const double *cX, const double *cY,
+                          const double *cCx, const double *cCy);
+/* Builds the same tables on the GPU in FP64: per interior cell and streaming
+ * direction an adaptive RK45 backtrace over dt, then the Fourier fit.
+ * Replaces: Radiation::UpdateFourierCoefficients (src/Radiation.cpp:191-237)
+ * with RK45_GeodesicEquation<-1> (src/GeodesicEquationSolver.cpp:78-188).
+ * metric is the per-cell block [N*RT_NMETRIC]; rk45_tol is the velocity
+ * truncation tolerance (the reference hard-codes 1e-4). */
+int rt_build_fourier_tables(rt_ctx *ctx, const double *metric, double rk45_tol);
+/* Copies the current tables back to the host ([N*nCoeff] each). */
+int rt_get_fourier_tables(rt_ctx *ctx, double *cS, double *cX, double *cY, double *cCx,
+                          double *cCy);
+/* Fluid fields (Radiation::eta, kappa0, kappa1, kappaA, ux, uy;
+ * src/Radiation.h:63-68), each [N].  Rejects NaN/Inf. */
+int rt_set_fields(rt_ctx *ctx, const double *eta, const double *kappa0, const double *kappa1,
+                  const double *kappaA, const double *ux, const double *uy);
+
+/* ---- state --------------------------------------------------------------- */
+/* Host AoS FP64 intensities [N*nDir] in/out (Radiation::I). */
+int rt_upload_I(rt_ctx *ctx, const double *I);
+int rt_download_I(rt_ctx *ctx, double *I);
+/* Inertial-frame moments E,Fx,Fy,Pxx,Pxy,Pyy (Radiation::E..Pyy), each [N]. */
+int rt_get_moments(rt_ctx *ctx, double *E, double *Fx, double *Fy, double *Pxx, double *Pxy,
+                   double *Pyy);
+/* Lambda-iteration statistics accumulated since the last reset: the maximum
+ * per-cell cycle count and the sum of all per-cell counts
+ * (Radiation::maxItterationCount / averageItterationCount numerator,
+ * src/Radiation.cpp:726-733). */
+int rt_iteration_stats(rt_ctx *ctx, int32_t *maxCount, double *sumCount);
+int rt_reset_iteration_stats(rt_ctx *ctx);
+
+/* ---- hot path ------------------------------------------------------------ */
+/* nSteps full steps, each stream + moments + collide fused in one kernel.
+ * Replaces: the loop body of Radiation::RunSimulation (src/Radiation.cpp:795-819)
+ * for StreamingType FlatFixed and CurvedFixed.  If want_moments is non-zero the
+ * moments of the last step are kept for rt_get_moments. */
+int rt_step(rt_ctx *ctx, int32_t nSteps, int32_t want_moments);
+/* Reference-order single methods (FP64 arithmetic, same operation order as
+ * the reference), each followed by the reference's buffer swap.
+ * Replace: Radiation::StreamFlatFixed / StreamCurvedFixed (:380-414, :454-512),
+ * ComputeMomentsIF (:239-269) and Collide (:641-736). */
+int rt_stream(rt_ctx *ctx);
+int rt_compute_moments(rt_ctx *ctx);
+int rt_collide(rt_ctx *ctx);
+/* Which implementation rt_step uses: 1 fused kernel, 0 reference-order
+ * kernels (for stencils or time steps the fused kernel does not cover). */
+int rt_uses_fused_path(rt_ctx *ctx, int32_t *fused);
+
+/* ---- timing helper ------------------------------------------------------- */
+/* Runs rt_step(ctx, nSteps, 0) bracketed by HIP events recorded on the
+ * context's own stream and returns the elapsed milliseconds. */
+int rt_time_steps(rt_ctx *ctx, int32_t nSteps, float *ms);
+
+/* ---- set-up helpers for callers without the C++ classes ------------------ */
+/* Stencil(nOrder, 0, isIntensityStencil) (src/Stencil.cpp:177-297):
+ * w, phi, cx, cy get nOrder entries; if Nc/Wc are non-null they get the
+ * InterpolationGrid(10*nOrder) cubic tables (nGrid = 10*nOrder, *4 each). */
+int rt_build_stencil(int32_t nOrder, double *w, double *phi, double *cx, double *cy,
+                     int32_t *Nc, double *Wc);
+/* Metric arrays of Minkowski (kind 0), SchwarzSchild (1) or KerrSchild (2)
+ * on a reference Grid(nx_, ny_, start, end) (src/Metric.cpp:126-336):
+ * fills the per-cell block [N*RT_NMETRIC] (metric may be NULL) and, if non-null,
+ * alpha and the spatial tetrad t11,t12,t21,t22 ([N] each). */
+int rt_build_metric(int32_t kind, double m, double a, int32_t nx, int32_t ny, double dx,
+                    double dy, double startx, double starty, double *metric, double *alpha,
+                    double *t11, double *t12, double *t21, double *t22);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

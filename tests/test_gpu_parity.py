@@ -1,0 +1,187 @@
+"""GPU parity: the HIP path against the CPU oracle on the same seeded inputs.
+
+Bars: FP64 paths match the oracle to rel-L2 <= 1e-12 (the reference-order
+kernels bitwise on the flat case); FP32-mixed matches the FP64 GPU path to
+rel-L2 <= 1e-5 on I, E, Fx, Fy.  All calls go through the C ABI.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_l2, oracle_problem, lib_context, grids, active_mask
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+def gaussian_fields(gspec, sigma=0.1, amp=1.0):
+    x, y = gspec.coords()
+    return {"eta": amp * np.exp(-(x * x + y * y) / (2 * sigma ** 2))}
+
+
+def mixed_fields(rt, gspec, seed, k0=1.0, k1=0.5, kA=0.2, u=0.0):
+    N = gspec.N
+    W = rt.workloads
+    x, y = gspec.coords()
+    f = {"eta": np.exp(-(x * x + y * y) / 0.08) + 0.1 * W.uniform(seed, N, 1),
+         "kappa0": k0 * W.uniform(seed, N, 2),
+         "kappa1": k1 * W.uniform(seed, N, 3),
+         "kappaA": kA * W.uniform(seed, N, 4)}
+    if u:
+        f["ux"] = u * (2 * W.uniform(seed, N, 5) - 1)
+        f["uy"] = u * (2 * W.uniform(seed, N, 6) - 1)
+    return f
+
+
+# ---------------------------------------------------------------- flat, FP64
+def test_flat_minkowski_100sq_d20_reference_order_bitwise(rt, O):
+    """Config: 100x100 on [-1,1]^2, 20 dirs, CFL 0.5, Gaussian eta, kappa 0, 100 steps."""
+    gs, og = grids(rt, O, 100, 100, (-1, -1), (1, 1))
+    f = gaussian_fields(gs)
+    P, _, _ = oracle_problem(O, og, 20, 0, f)
+    Iref, mref, mx, sm, _ = P.run(np.zeros(og.N * 20), 100)
+    ctx, _ = lib_context(rt, gs, 20, 0, f, rt.FP64)
+    for _ in range(100):
+        ctx.stream()
+        ctx.compute_moments()
+        ctx.collide()
+    I = ctx.download_I()
+    assert np.array_equal(I, Iref), f"max abs diff {np.abs(I - Iref).max()}"
+    assert np.array_equal(ctx.moments(), mref)
+    gmx, gsm = ctx.iteration_stats()
+    assert (gmx, gsm) == (mx, sm)
+
+
+def test_flat_minkowski_100sq_d20_fused_fp64(rt, O):
+    gs, og = grids(rt, O, 100, 100, (-1, -1), (1, 1))
+    f = gaussian_fields(gs)
+    P, _, _ = oracle_problem(O, og, 20, 0, f)
+    Iref, mref, mx, sm, _ = P.run(np.zeros(og.N * 20), 100)
+    ctx, _ = lib_context(rt, gs, 20, 0, f, rt.FP64)
+    assert ctx.uses_fused()
+    ctx.step(100)
+    I = ctx.download_I()
+    assert rel_l2(I, Iref) <= TOL64
+    act = active_mask(gs, -1.0)
+    m = ctx.moments()
+    for X in range(6):
+        assert rel_l2(m[X][act], mref[X][act]) <= 1e-10
+    gmx, gsm = ctx.iteration_stats()
+    assert (gmx, gsm) == (mx, sm)
+
+
+def test_flat_fp32_mixed_vs_fp64(rt, O):
+    gs, _ = grids(rt, O, 100, 100, (-1, -1), (1, 1))
+    f = gaussian_fields(gs)
+    c64, _ = lib_context(rt, gs, 20, 0, f, rt.FP64)
+    c32, _ = lib_context(rt, gs, 20, 0, f, rt.FP32_MIXED)
+    c64.step(100)
+    c32.step(100)
+    I64, I32 = c64.download_I(), c32.download_I()
+    assert rel_l2(I32, I64) <= TOL32
+    m64, m32 = c64.moments(), c32.moments()
+    for X in range(3):
+        assert rel_l2(m32[X], m64[X]) <= TOL32
+
+
+@pytest.mark.parametrize("nDir,nx_,ny_", [(16, 70, 33), (24, 65, 9), (48, 130, 20), (64, 2, 2), (32, 64, 64)])
+def test_flat_collisions_shapes_fp64(rt, O, nDir, nx_, ny_):
+    """Ragged strips, tiny grids and every fused stencil size, with scattering."""
+    gs, og = grids(rt, O, nx_, ny_, (-1, -0.5), (1, 0.5), cfl=0.7)
+    f = mixed_fields(rt, gs, 7 + nDir)
+    P, _, _ = oracle_problem(O, og, nDir, 0, f)
+    I0 = rt.workloads.uniform(99, og.N * nDir)
+    Iref, mref, mx, sm, _ = P.run(I0, 7)
+    ctx, _ = lib_context(rt, gs, nDir, 0, f, rt.FP64)
+    ctx.upload_I(I0)
+    assert ctx.uses_fused()
+    ctx.step(7)
+    I = ctx.download_I()
+    assert rel_l2(I, Iref) <= TOL64
+    gmx, gsm = ctx.iteration_stats()
+    assert gmx == mx and abs(gsm - sm) <= 0.001 * sm + 2
+
+
+def test_flat_moving_fluid_general_collide(rt, O):
+    """u != 0 exercises the per-direction lambda cycle with a moment exchange."""
+    gs, og = grids(rt, O, 40, 30, (-1, -1), (1, 1))
+    f = mixed_fields(rt, gs, 5, u=0.5)
+    P, _, _ = oracle_problem(O, og, 32, 0, f)
+    Iref, mref, mx, sm, _ = P.run(np.zeros(og.N * 32), 10)
+    for fused in (True, False):
+        ctx, _ = lib_context(rt, gs, 32, 0, f, rt.FP64)
+        if fused:
+            ctx.step(10)
+        else:
+            for _ in range(10):
+                ctx.stream()
+                ctx.compute_moments()
+                ctx.collide()
+        assert rel_l2(ctx.download_I(), Iref) <= TOL64
+        gmx, gsm = ctx.iteration_stats()
+        assert gmx == mx
+
+
+# -------------------------------------------------------------- curved, FP64
+@pytest.mark.parametrize("kind,tol", [(1, 1e-4), (2, 1e-4), (2, 1e-10)])
+def test_geodesic_tables_gpu_vs_oracle(rt, O, kind, tol):
+    gs, og = grids(rt, O, 40, 36, (-4, -4), (4, 4))
+    bh = 2.1 * 2.1
+    M = O.metric(og, kind)
+    ref = O.geodesic_tables(og, M, bh, tol)
+    ctx, _ = lib_context(rt, gs, 16, kind, {}, rt.FP64, bh_r2=bh, tol=tol)
+    got = ctx.get_fourier_tables()
+    for a, b in zip(got, ref):
+        assert rel_l2(a, b) <= 1e-12
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_curved_reference_order_vs_oracle(rt, O, kind):
+    """Oracle tables uploaded, per-method kernels: same operation order."""
+    gs, og = grids(rt, O, 48, 40, (-4, -4), (4, 4))
+    bh = 2.1 * 2.1
+    f = mixed_fields(rt, gs, 11 + kind, k0=0.8, k1=0.3, kA=0.0)
+    P, M, tabs = oracle_problem(O, og, 32, kind, f, bh_r2=bh)
+    Iref, mref, mx, sm, _ = P.run(np.zeros(og.N * 32), 12)
+    ctx, _ = lib_context(rt, gs, 32, kind, f, rt.FP64, bh_r2=bh, tables=tabs)
+    for _ in range(12):
+        ctx.stream()
+        ctx.compute_moments()
+        ctx.collide()
+    assert rel_l2(ctx.download_I(), Iref) <= TOL64
+    gmx, gsm = ctx.iteration_stats()
+    assert gmx == mx and abs(gsm - sm) <= 1e-3 * sm
+
+
+@pytest.mark.parametrize("kind,nDir", [(1, 32), (2, 16), (1, 64)])
+def test_curved_fused_fp64_vs_oracle(rt, O, kind, nDir):
+    gs, og = grids(rt, O, 70, 44, (-4, -4), (4, 4))
+    bh = 2.1 * 2.1
+    f = mixed_fields(rt, gs, 21 + kind, k0=1.0, k1=0.3, kA=0.0)
+    P, M, tabs = oracle_problem(O, og, nDir, kind, f, bh_r2=bh)
+    Iref, mref, mx, sm, _ = P.run(np.zeros(og.N * nDir), 15)
+    ctx, _ = lib_context(rt, gs, nDir, kind, f, rt.FP64, bh_r2=bh)  # GPU-built tables
+    assert ctx.uses_fused()
+    ctx.step(15)
+    I = ctx.download_I()
+    assert rel_l2(I, Iref) <= TOL64
+    act = active_mask(gs, bh)
+    m = ctx.moments()
+    for X in range(3):
+        assert rel_l2(m[X][act], mref[X][act]) <= 1e-10
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_curved_fp32_mixed_vs_fp64(rt, O, kind):
+    gs, _ = grids(rt, O, 96, 80, (-4, -4), (4, 4))
+    bh = 2.1 * 2.1
+    f = mixed_fields(rt, gs, 31 + kind, k0=1.0, k1=0.3, kA=0.0)
+    c64, _ = lib_context(rt, gs, 32, kind, f, rt.FP64, bh_r2=bh)
+    c32, _ = lib_context(rt, gs, 32, kind, f, rt.FP32_MIXED, bh_r2=bh)
+    c64.step(60)
+    c32.step(60)
+    assert rel_l2(c32.download_I(), c64.download_I()) <= TOL32
+    m64, m32 = c64.moments(), c32.moments()
+    for X in range(3):
+        assert rel_l2(m32[X], m64[X]) <= TOL32
