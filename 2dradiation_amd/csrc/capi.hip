@@ -326,9 +326,16 @@ int rt_set_geometry(rt_ctx *c, const double *alpha, const double *t11, const dou
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
     NEED(c, alpha); NEED(c, t11); NEED(c, t12); NEED(c, t21); NEED(c, t22);
     const size_t N = c->N;
-    if (!all_finite(alpha, N)) return rt_fail(c, RT_ERR_NONFINITE, "alpha contains NaN/Inf");
-    for (size_t k = 0; k < N; k++)
+    const rt_desc &d = c->desc;
+    for (size_t k = 0; k < N; k++) {
+        /* cells inside the excision radius are never streamed from or collided */
+        const double x = d.startx + (double)(k % d.nx) * d.dx, y = d.starty + (double)(k / d.nx) * d.dy;
+        if (d.excision_r2 >= 0.0 && x * x + y * y <= d.excision_r2) continue;
+        if (!std::isfinite(alpha[k]) || !std::isfinite(t11[k]) || !std::isfinite(t12[k]) || !std::isfinite(t21[k]) ||
+            !std::isfinite(t22[k]))
+            return rt_fail(c, RT_ERR_NONFINITE, "lapse or tetrad contains NaN/Inf outside the excision region");
         if (!(alpha[k] > 0)) return rt_fail(c, RT_ERR_ARG, "alpha must be positive");
+    }
     RT_HIP(c, hipMemcpyAsync(c->d_alpha, alpha, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     RT_HIP(c, hipMemcpyAsync(c->d_t11, t11, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     RT_HIP(c, hipMemcpyAsync(c->d_t12, t12, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -367,8 +374,9 @@ int rt_build_fourier_tables(rt_ctx *c, const double *metric, double tol)
     NEED(c, metric);
     if (!c->has_sstencil) return rt_fail(c, RT_ERR_STATE, "rt_set_streaming_stencil must come first");
     if (!(tol > 0) || !std::isfinite(tol)) return rt_fail(c, RT_ERR_ARG, "rk45 tolerance must be positive");
+    /* the block may hold NaN at a coordinate singularity inside the excision
+     * region (Kerr-Schild at r = 0); the tables are validated instead */
     const size_t bytes = (size_t)c->N * RT_NMETRIC * sizeof(double);
-    if (!all_finite(metric, (size_t)c->N * RT_NMETRIC)) return rt_fail(c, RT_ERR_NONFINITE, "metric block contains NaN/Inf");
     double *dm = nullptr;
     TRY(dalloc(c, &dm, bytes));
     hipError_t e = hipMemcpyAsync(dm, metric, bytes, hipMemcpyHostToDevice, c->stream);
@@ -385,6 +393,10 @@ int rt_build_fourier_tables(rt_ctx *c, const double *metric, double tol)
     c->has_tables = true;
     TRY(refresh_departure_bound(c));
     RT_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->has_stencil && !(c->max_disp < 1e29)) {
+        c->has_tables = false;
+        return rt_fail(c, RT_ERR_NONFINITE, "geodesic tables are not finite at an active cell (metric NaN/Inf near the path?)");
+    }
     return RT_OK;
 }
 

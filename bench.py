@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--ref-steps", type=int, default=200,
+                    help="steps of the FP64 reference-order comparison leg (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -146,7 +148,6 @@ def main():
         ms64 = c64.time_steps(K)
         rate64 = upd_per_step * K / (ms64 * 1e-3)
         # quality: both runs started from I=0 and took K steps
-        c32.step(0)
         I32, I64 = c32.download_I(), c64.download_I()
         # moments of the final state (one more step with moments requested on copies is not
         # needed: compare the state's moments computed on the host from I)
@@ -166,6 +167,35 @@ def main():
         result["rel_l2_fp32_vs_fp64"] = {"I": rel_l2(I32, I64), "E": rel_l2(E32, E64),
                                          "Fx": rel_l2(Fx32, Fx64), "Fy": rel_l2(Fy32, Fy64)}
         c64.close()
+        if args.ref_steps > 0:
+            # FP64 reference-order kernels (the oracle's operation order) as the
+            # FP64 truth; compared after ref_steps steps from I = 0
+            R = min(args.ref_steps, K)
+            print(f"[bench] FP64 reference-order leg: {R} steps", file=sys.stderr, flush=True)
+            cr, _ = rt.workloads.make_context(WORKLOAD, rt.FP64)
+            t0 = time.perf_counter()
+            for _ in range(R):
+                cr.stream()
+                cr.compute_moments()
+                cr.collide()
+            cr.synchronize()
+            tref = time.perf_counter() - t0
+            Iref = cr.download_I()
+            cr.close()
+            a32, _ = rt.workloads.make_context(WORKLOAD, rt.FP32_MIXED)
+            a32.step(R, want_moments=False)
+            a64, _ = rt.workloads.make_context(WORKLOAD, rt.FP64)
+            a64.step(R, want_moments=False)
+            J32, J64 = a32.download_I(), a64.download_I()
+            a32.close()
+            a64.close()
+            Er, Fxr, Fyr = mom(Iref)
+            Ea, Fxa, Fya = mom(J32)
+            Eb, Fxb, Fyb = mom(J64)
+            result["rel_l2_vs_reference_order_fp64"] = {
+                "steps": R, "ms_per_step_reference_order": 1e3 * tref / R,
+                "fp32_fused": {"I": rel_l2(J32, Iref), "E": rel_l2(Ea, Er), "Fx": rel_l2(Fxa, Fxr), "Fy": rel_l2(Fya, Fyr)},
+                "fp64_fused": {"I": rel_l2(J64, Iref), "E": rel_l2(Eb, Er), "Fx": rel_l2(Fxb, Fxr), "Fy": rel_l2(Fyb, Fyr)}}
     c32.close()
 
     if not args.no_cpu:
