@@ -11,6 +11,7 @@
 #include "../cpp/Stencil.h"
 
 int launch_mask(rt_ctx *c);
+static int ensure_unscaled(rt_ctx *c);
 
 namespace {
 thread_local std::string g_create_error;
@@ -336,6 +337,7 @@ int rt_set_geometry(rt_ctx *c, const double *alpha, const double *t11, const dou
             return rt_fail(c, RT_ERR_NONFINITE, "lapse or tetrad contains NaN/Inf outside the excision region");
         if (!(alpha[k] > 0)) return rt_fail(c, RT_ERR_ARG, "alpha must be positive");
     }
+    TRY(ensure_unscaled(c));
     RT_HIP(c, hipMemcpyAsync(c->d_alpha, alpha, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     RT_HIP(c, hipMemcpyAsync(c->d_t11, t11, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     RT_HIP(c, hipMemcpyAsync(c->d_t12, t12, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -432,6 +434,7 @@ int rt_set_fields(rt_ctx *c, const double *eta, const double *kappa0, const doub
         if (ux[k] != 0.0 || uy[k] != 0.0) any_u = true;
         if (ux[k] * ux[k] + uy[k] * uy[k] >= 1.0) return rt_fail(c, RT_ERR_ARG, "fluid velocity must be below the speed of light");
     }
+    TRY(ensure_unscaled(c));
     for (int q = 0; q < 6; q++)
         RT_HIP(c, hipMemcpyAsync(c->d_f64 + q * N, src[q], N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     c->any_u = any_u;
@@ -463,6 +466,7 @@ int rt_upload_I(rt_ctx *c, const double *I)
     }
     dfree(tmp);
     if (r == RT_OK) c->has_I = true;
+    c->scaled = false;
     return r;
 }
 
@@ -471,6 +475,7 @@ int rt_download_I(rt_ctx *c, double *I)
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
     NEED(c, I);
     const size_t n = (size_t)c->N * c->D;
+    TRY(ensure_unscaled(c));
     double *tmp = nullptr;
     TRY(dalloc(c, &tmp, n * sizeof(double)));
     int r = launch_transpose_out(c, c->d_I[c->cur], tmp);
@@ -516,6 +521,17 @@ int rt_reset_iteration_stats(rt_ctx *c)
     return RT_OK;
 }
 
+/* Brings d_I[cur] back to intensities if the fused curved path left it as
+ * alpha^-3 I, and makes the fixed cells of both buffers equal again. */
+static int ensure_unscaled(rt_ctx *c)
+{
+    if (!c->scaled) return RT_OK;
+    TRY(launch_scale_state(c, c->d_I[c->cur], true));
+    TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+    c->scaled = false;
+    return RT_OK;
+}
+
 static int check_ready(rt_ctx *c)
 {
     if (!c->has_stencil) return rt_fail(c, RT_ERR_STATE, "rt_set_stencil has not been called");
@@ -541,6 +557,10 @@ int rt_step(rt_ctx *c, int32_t nSteps, int32_t want_moments)
     TRY(check_ready(c));
     if (nSteps == 0) return RT_OK;
     if (fused_supported(c)) {
+        if (c->desc.streaming == RT_CURVED_FIXED && !c->scaled) {
+            TRY(launch_scale_state(c, c->d_I[c->cur], false));
+            c->scaled = true;
+        }
         /* halo and excised cells are the same in both buffers from here on */
         TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
         if (want_moments) TRY(launch_fill(c, c->d_mom, 6 * (size_t)c->N, 0.0));
@@ -549,6 +569,7 @@ int rt_step(rt_ctx *c, int32_t nSteps, int32_t want_moments)
             c->cur = 1 - c->cur;
         }
     } else {
+        TRY(ensure_unscaled(c));
         for (int n = 0; n < nSteps; n++) {
             TRY(launch_ref_stream(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
             c->cur = 1 - c->cur;
@@ -565,6 +586,7 @@ int rt_stream(rt_ctx *c)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
     TRY(check_ready(c));
+    TRY(ensure_unscaled(c));
     TRY(launch_ref_stream(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
     c->cur = 1 - c->cur;
     return RT_OK;
@@ -573,12 +595,14 @@ int rt_compute_moments(rt_ctx *c)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
     TRY(check_ready(c));
+    TRY(ensure_unscaled(c));
     return launch_ref_moments(c, c->d_I[c->cur]);
 }
 int rt_collide(rt_ctx *c)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
     TRY(check_ready(c));
+    TRY(ensure_unscaled(c));
     TRY(launch_ref_collide(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
     c->cur = 1 - c->cur;
     return RT_OK;

@@ -61,51 +61,146 @@ __device__ inline float fast_div<float>(float a, float b)
     return a * __builtin_amdgcn_rcpf(b);
 }
 
-/* Fine angular index nGrid*phi/(2 pi) of Tensor2::Phi (src/DataTypes.hh:166-170)
- * with the Hastings atan2 (src/Utility.hh:220-254).  The octant offsets are
- * exact multiples of nGrid/4, so only the polynomial term is rounded. */
-template <typename T>
-__device__ inline T angle_index(T vx, T vy, T u, int quarter, int nGrid)
-{
-    if (vx != T(0)) {
-        if (fabsf((float)vx) > fabsf((float)vy)) {
-            T p = rtm::hastings_atan(fast_div(vy, vx));
-            if (vx > T(0)) return p >= T(0) ? u * p : u * p + T(nGrid);
-            return u * p + T(2 * quarter);
-        }
-        T p = rtm::hastings_atan(fast_div(vx, vy));
-        if (vy > T(0)) return T(quarter) - u * p;
-        return T(3 * quarter) - u * p;
-    }
-    if (vy > T(0)) return T(quarter);
-    if (vy < T(0)) return T(3 * quarter);
-    return T(0);
-}
-
 template <typename T>
 __device__ inline T tmax0(T v)
 {
     return v > T(0) ? v : T(0);
 }
 
+/* Per-direction tables read with wave-uniform indices: a pointer into the
+ * constant address space lets the compiler use scalar loads. */
+typedef const __attribute__((address_space(4))) DirTables CDirTables;
+
+/* LDS of one workgroup.  Intensity rows are SoA [direction][66 columns]
+ * with 4 wrap-around direction rows (D..D+3 repeat 0..3), so the 5 angular
+ * taps of a corner are 5 rows of one column at constant offsets; lanes
+ * (= columns) hit consecutive banks.  Rows and FP64 coefficients arrive by
+ * direct global-to-LDS loads, without passing through registers. */
 template <typename T, int D, bool CURVED>
 struct Smem {
-    T I[3][D][66];
-    T C[CURVED ? 2 : 1][CURVED ? 25 : 1][64];
+    static constexpr int RW = 66;
+    T I[3][D + 4][RW];
+    double C[CURVED ? 25 : 1][64];
     double X[RT_FUSED_WAVES][6][64];
     T wA[10][5], wD[10][5];
     int woff[10];
 };
 
+/* Copies n dwords from global src to LDS dst with one dword per lane per
+ * instruction; lanes past n do nothing.  dst must be wave-uniform. */
+__device__ inline void lds_dma_dwords(const void *src, void *dst, int n, int lane)
+{
+    const uint32_t *g = (const uint32_t *)src;
+    uint32_t *l = (uint32_t *)dst;
+    for (int b = 0; b < n; b += 64) {
+        if (b + lane < n) __builtin_amdgcn_global_load_lds(g + b + lane, l + b, 4, 0, 0);
+    }
+}
+
+/* Fine angular index nGrid*phi/(2 pi) of Tensor2::Phi (src/DataTypes.hh:166-170)
+ * with the Hastings atan2 (src/Utility.hh:220-254), branch-free.  The octant
+ * offsets are exact multiples of nGrid/4, so only the polynomial term rounds. */
+template <typename T>
+__device__ inline T angle_index_bf(T vx, T vy, T u, int quarter)
+{
+    const bool swap = !(fabsf((float)vx) > fabsf((float)vy));
+    const T num = swap ? vx : vy, den = swap ? vy : vx;
+    const T p = rtm::hastings_atan(fast_div(num, den));
+    const T up = u * p;
+    const T off_a = vx > T(0) ? (p < T(0) ? T(4 * quarter) : T(0)) : T(2 * quarter);
+    const T off_b = vy > T(0) ? T(quarter) : T(3 * quarter);
+    T dd = swap ? off_b - up : off_a + up;
+    return (vx == T(0) && vy == T(0)) ? T(0) : dd;
+}
+
+/* Reference lambda cycle for rows that contain a moving cell
+ * (src/Radiation.cpp:661-722), with a workgroup moment exchange per cycle.
+ * Every wavefront evaluates the same per-cell decisions, so the trip count is
+ * uniform across the workgroup. */
+template <typename T, int DW>
+__device__ inline void collide_moving(T (&sv)[DW], double (&mOut)[6], int &cycles, const double (&mI)[6],
+                                      bool active, double eta, double k0, double k1, double kA, double u1,
+                                      double u2, double adt, int wv, int lane, CDirTables *dtab,
+                                      double (*X)[6][64])
+{
+    const double uu = u1 * u1 + u2 * u2;
+    const double lor = 1.0 / sqrt(1.0 - uu);
+    double prev[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) prev[q] = mI[q];
+    double diff = 1;
+    T fin[DW];
+#pragma unroll
+    for (int k = 0; k < DW; k++) fin[k] = sv[k];
+    while (true) {
+        const bool iter = active && fabs(diff) > 1e-4 && cycles < 100;
+        if (!__any(iter)) break;
+        double part[6] = {0, 0, 0, 0, 0, 0};
+        if (iter) {
+            cycles++;
+            const double uF = u1 * prev[1] + u2 * prev[2];
+            const double uxP = u1 * prev[3] + u2 * prev[4];
+            const double uyP = u1 * prev[4] + u2 * prev[5];
+            const double uuP = u1 * uxP + u2 * uyP;
+            const double fE = lor * lor * (prev[0] - 2.0 * uF + uuP);
+            const double fFx = lor * lor * (prev[1] - lor * prev[0] * u1 - uxP +
+                                            u1 * lor / (1.0 + lor) * ((2.0 * lor + 1.0) * uF - lor * uuP));
+            const double fFy = lor * lor * (prev[2] - lor * prev[0] * u2 - uyP +
+                                            u2 * lor / (1.0 + lor) * ((2.0 * lor + 1.0) * uF - lor * uuP));
+#pragma unroll
+            for (int k = 0; k < DW; k++) {
+                const int d = wv * DW + k;
+                const double n1 = dtab->cx[d], n2 = dtab->cy[d];
+                const double un = u1 * n1 + u2 * n2;
+                const double A = lor * (1.0 - un);
+                const double fac = (1 - (lor * un) / (lor + 1)) * lor;
+                const double nF = ((n1 - fac * u1) / A) * fFx + ((n2 - fac * u2) / A) * fFy;
+                const double M = k0 * fE + 3.0 * k1 * nF;
+                const double v = ((double)sv[k] + adt * (eta + M) / (A * A)) / (1.0 + adt * (kA + k0) * A);
+                fin[k] = (T)v;
+#pragma unroll
+                for (int q = 0; q < 6; q++) part[q] += dtab->w6[d][q] * v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 6; q++) X[wv][q][lane] = part[q];
+        __syncthreads();
+        if (iter) {
+            double ss = 0;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const double nw = ((X[0][q][lane] + X[1][q][lane]) + X[2][q][lane]) + X[3][q][lane];
+                const double r = (prev[q] - nw) / nw;
+                ss += r * r;
+                prev[q] = nw;
+            }
+            diff = sqrt(ss);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) mOut[q] = prev[q];
+#pragma unroll
+    for (int k = 0; k < DW; k++) sv[k] = fin[k];
+}
+
+/* two workgroups per CU where their LDS fits in 160 KB */
 template <typename T, int D, bool CURVED>
-__global__ __launch_bounds__(256) void k_fused(KP<T> p)
+constexpr int fused_min_blocks()
+{
+    return sizeof(Smem<T, D, CURVED>) <= 80 * 1024 ? 2 : 1;
+}
+
+template <typename T, int D, bool CURVED>
+__global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fused(KP<T> p)
 {
     constexpr int DW = D / RT_FUSED_WAVES;
+    constexpr int RW = Smem<T, D, CURVED>::RW;
     __shared__ Smem<T, D, CURVED> sm;
 
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const DirTables *__restrict__ dtab = p.dt;
+    CDirTables *dtab0 = (CDirTables *)p.dt;
     const long N = p.N;
     const int nx = p.nx;
 
@@ -119,119 +214,99 @@ __global__ __launch_bounds__(256) void k_fused(KP<T> p)
     const int chunk = tile % p.chunks;
     const int col0 = 64 * strip;                /* grid column of LDS column 0 */
     const int gi = col0 + 1 + lane;             /* this lane's cell column */
+    const bool inside = gi <= nx - 2;
     const int rowA = 1 + chunk * p.rows_per_block;
     const int rowB = min(rowA + p.rows_per_block, p.ny - 1); /* exclusive */
     if (rowA >= rowB) return;
+    const int ncols = min(RW, nx - col0);       /* valid LDS columns */
 
     if (CURVED) {
         if (threadIdx.x < 50) {
             int r = threadIdx.x / 5, m = threadIdx.x % 5;
             if constexpr (sizeof(T) == 4) {
-                sm.wA[r][m] = dtab->wAf[r][m];
-                sm.wD[r][m] = dtab->wDf[r][m];
+                sm.wA[r][m] = dtab0->wAf[r][m];
+                sm.wD[r][m] = dtab0->wDf[r][m];
             } else {
-                sm.wA[r][m] = dtab->wA[r][m];
-                sm.wD[r][m] = dtab->wD[r][m];
+                sm.wA[r][m] = dtab0->wA[r][m];
+                sm.wD[r][m] = dtab0->wD[r][m];
             }
         }
-        if (threadIdx.x < 10) sm.woff[threadIdx.x] = dtab->woff[threadIdx.x];
+        if (threadIdx.x < 10) sm.woff[threadIdx.x] = dtab0->woff[threadIdx.x];
     }
 
-    /* ---- staging helpers ---- */
-    T preg[DW], preg2[DW];
-    double creg[7];
-    auto load_row = [&](int row) {
-        const long rbase = (long)row * nx;
-        const int c = col0 + lane, c2 = col0 + 64 + lane;
-        T s1 = T(1), s2 = T(1);
-        if (CURVED) {
-            if (c < nx) s1 = p.cell[CF_IA3 * N + rbase + c];
-            if (lane < 2 && c2 < nx) s2 = p.cell[CF_IA3 * N + rbase + c2];
-        }
-#pragma unroll
-        for (int k = 0; k < DW; k++) {
-            const int d = wv * DW + k;
-            const T *src = p.Iin + (long)d * N + rbase;
-            preg[k] = (c < nx) ? src[c] * s1 : T(0);
-            preg2[k] = (lane < 2 && c2 < nx) ? src[c2] * s2 : T(0);
-        }
-    };
-    auto store_row = [&](int row) {
+    /* ---- asynchronous staging (direct global -> LDS) ---- */
+    constexpr int RDW = (int)(RW * sizeof(T) / 4); /* dwords per LDS row */
+    auto stage_row = [&](int row) {
         const int slot = row % 3;
-#pragma unroll
-        for (int k = 0; k < DW; k++) {
-            const int d = wv * DW + k;
-            sm.I[slot][d][lane] = preg[k];
-            if (lane < 2) sm.I[slot][d][64 + lane] = preg2[k];
+        const int ndw = (int)(ncols * sizeof(T) / 4);
+        for (int r = wv; r < D + 4; r += RT_FUSED_WAVES) {
+            const int d = r < D ? r : r - D;
+            const T *src = p.Iin + (size_t)d * N + (size_t)row * nx + col0;
+            lds_dma_dwords(src, &sm.I[slot][r][0], ndw, lane);
         }
+        (void)RDW;
     };
-    auto load_coef = [&](int row) {
+    auto stage_coef = [&](int row) {
         if (!CURVED) return;
-        const long ij = (long)row * nx + gi;
-        const bool ok = gi <= nx - 2;
-#pragma unroll
-        for (int q = 0; q < 7; q++) {
-            const int e = wv + 4 * q;
-            creg[q] = (e < 25 && ok) ? p.coef[(long)e * N + ij] : 0.0;
-        }
-    };
-    auto store_coef = [&](int row) {
-        if (!CURVED) return;
-        const double xc = p.startx + gi * p.dx, yc = p.starty + row * p.dy;
-        const double idx = 1.0 / p.dx, idy = 1.0 / p.dy;
-#pragma unroll
-        for (int q = 0; q < 7; q++) {
-            const int e = wv + 4 * q;
-            if (e < 25) {
-                double v = creg[q];
-                const int ser = e / 5, k = e % 5;
-                if (ser == 1) v = (k == 0 ? v - xc : v) * idx;
-                if (ser == 2) v = (k == 0 ? v - yc : v) * idy;
-                sm.C[row & 1][e][lane] = (T)v;
-            }
+        const int n = (int)(min(64, nx - 1 - (col0 + 1)) * 2); /* interior cells of the strip */
+        for (int e = wv; e < 25; e += RT_FUSED_WAVES) {
+            const double *src = p.coef + (size_t)e * N + (size_t)row * nx + col0 + 1;
+            lds_dma_dwords(src, &sm.C[e][0], n, lane);
         }
     };
 
     /* prologue: rows rowA-1, rowA, rowA+1 and the coefficients of rowA */
-    for (int r = rowA - 1; r <= rowA + 1; r++) {
-        load_row(r);
-        store_row(r);
-    }
-    load_coef(rowA);
-    store_coef(rowA);
+    for (int r = rowA - 1; r <= rowA + 1; r++) stage_row(r);
+    stage_coef(rowA);
 
     unsigned long long it_sum = 0, it_max = 0;
-    const double *w6 = &dtab->w6[0][0];
+    const double dtime = p.dtime;
 
     for (int j = rowA; j < rowB; j++) {
-        __syncthreads(); /* slots j-1..j+1 and coefficients of row j are visible */
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads(); /* slots j-1..j+1 and the coefficients of row j are in LDS */
         const bool more = (j + 1 < rowB);
-        if (more) {
-            load_row(j + 2);
-            load_coef(j + 1);
-        }
 
-        const long ij = (long)j * nx + gi;
-        const bool inside = gi <= nx - 2;
+        const unsigned ij = (unsigned)j * (unsigned)nx + (unsigned)gi;
         const uint8_t mk = inside ? p.mask[ij] : 0;
         const bool active = (mk & MASK_ACTIVE) != 0;
+        /* per-cell inputs, consumed by the collide below */
+        T f_eta = 0, f_k0 = 0, f_k1 = 0, f_kA = 0, f_ux = 0, f_uy = 0, f_adt = 0, f_ia3 = 1;
+        if (active) {
+            f_eta = (p.cell + (size_t)CF_ETA * N)[ij];
+            f_k0 = (p.cell + (size_t)CF_K0 * N)[ij];
+            f_k1 = (p.cell + (size_t)CF_K1 * N)[ij];
+            f_kA = (p.cell + (size_t)CF_KA * N)[ij];
+            f_ux = (p.cell + (size_t)CF_UX * N)[ij];
+            f_uy = (p.cell + (size_t)CF_UY * N)[ij];
+            f_adt = (p.cell + (size_t)CF_ADT * N)[ij];
+            if (CURVED) f_ia3 = (p.cell + (size_t)CF_IA3 * N)[ij];
+        }
+        /* An opaque copy of the table pointer per row keeps the per-direction
+         * constants from being hoisted out of the row loop. */
+        CDirTables *dtab = dtab0;
+        asm volatile("" : "+s"(dtab));
 
         /* ---- stream ---- */
         T sv[DW];
         double acc[6] = {0, 0, 0, 0, 0, 0};
         if (CURVED) {
+            /* geodesic Fourier coefficients of this cell; X and Y relative to
+             * the cell centre in cell units */
             T cS[5], cX[5], cY[5], cU[5], cV[5];
+            {
+                const double xc = p.startx + gi * p.dx, yc = p.starty + j * p.dy;
+                const double idx = 1.0 / p.dx, idy = 1.0 / p.dy;
 #pragma unroll
-            for (int k = 0; k < 5; k++) {
-                cS[k] = sm.C[j & 1][k][lane];
-                cX[k] = sm.C[j & 1][5 + k][lane];
-                cY[k] = sm.C[j & 1][10 + k][lane];
-                cU[k] = sm.C[j & 1][15 + k][lane];
-                cV[k] = sm.C[j & 1][20 + k][lane];
+                for (int k = 0; k < 5; k++) {
+                    cS[k] = (T)sm.C[k][lane];
+                    cX[k] = (T)((k == 0 ? sm.C[5][lane] - xc : sm.C[5 + k][lane]) * idx);
+                    cY[k] = (T)((k == 0 ? sm.C[10][lane] - yc : sm.C[10 + k][lane]) * idy);
+                    cU[k] = (T)sm.C[15 + k][lane];
+                    cV[k] = (T)sm.C[20 + k][lane];
+                }
             }
-            const T a3 = active ? T(1) / p.cell[CF_IA3 * N + ij] : T(0);
             const bool near = (mk & MASK_NEAR_BH) != 0;
-            const double xc = p.startx + gi * p.dx, yc = p.starty + j * p.dy;
             const T angu = (sizeof(T) == 4) ? (T)dtab->ang_uf : (T)dtab->ang_u;
             const int quarter = dtab->quarter, nGrid = dtab->nGrid;
 #pragma unroll
@@ -249,51 +324,49 @@ __global__ __launch_bounds__(256) void k_fused(KP<T> p)
                     vx += cU[q] * B[q];
                     vy += cV[q] * B[q];
                 }
-                T val = T(0);
                 bool go = active;
-                if (go && near) {
-                    const double xd = xc + (double)di * p.dx, yd = yc + (double)dj * p.dy;
+                if (near) {
+                    const double xd = p.startx + gi * p.dx + (double)di * p.dx;
+                    const double yd = p.starty + j * p.dy + (double)dj * p.dy;
                     if (xd * xd + yd * yd <= p.r2) go = false;
                 }
+                T val = T(0);
                 if (go) {
                     T fi = floor(di), fj = floor(dj);
                     fi = fi < T(-1) ? T(-1) : (fi > T(0) ? T(0) : fi);
                     fj = fj < T(-1) ? T(-1) : (fj > T(0) ? T(0) : fj);
                     const T fx = di - fi, fy = dj - fj;
-                    T dd = angle_index<T>(vx, vy, angu, quarter, nGrid);
+                    T dd = angle_index_bf<T>(vx, vy, angu, quarter);
                     dd = dd < T(1e-8) ? T(1e-8) : dd;
                     dd = dd > T(nGrid) - T(1e-8) ? T(nGrid) - T(1e-8) : dd;
                     int g0 = (int)dd;
                     const T t = dd - T(g0);
-                    if (g0 >= nGrid) g0 -= nGrid;
+                    g0 = (unsigned)g0 < (unsigned)nGrid ? g0 : 0; /* NaN-safe LDS index */
                     const int r = g0 % 10;
                     int s0 = g0 / 10 + sm.woff[r];
                     s0 = s0 < 0 ? s0 + D : (s0 >= D ? s0 - D : s0);
                     T Cw[5];
-                    int dirs[5];
 #pragma unroll
-                    for (int m = 0; m < 5; m++) {
-                        Cw[m] = sm.wA[r][m] + t * sm.wD[r][m];
-                        int dm = s0 + m;
-                        dirs[m] = dm >= D ? dm - D : dm;
-                    }
-                    const int sa = (j + (int)fj + 3) % 3, sb = (j + (int)fj + 4) % 3;
-                    const int ca = lane + 1 + (int)fi, cb = ca + 1;
+                    for (int m = 0; m < 5; m++) Cw[m] = sm.wA[r][m] + t * sm.wD[r][m];
+                    const int ra = (j + (int)fj + 3) % 3, rb = (j + (int)fj + 4) % 3;
+                    const int ca = lane + 1 + (int)fi;
+                    const T *pa = &sm.I[ra][s0][ca];
+                    const T *pb = &sm.I[rb][s0][ca];
                     T v00 = 0, v01 = 0, v10 = 0, v11 = 0;
 #pragma unroll
                     for (int m = 0; m < 5; m++) {
-                        v00 += Cw[m] * sm.I[sa][dirs[m]][ca];
-                        v01 += Cw[m] * sm.I[sb][dirs[m]][ca];
-                        v10 += Cw[m] * sm.I[sa][dirs[m]][cb];
-                        v11 += Cw[m] * sm.I[sb][dirs[m]][cb];
+                        v00 += Cw[m] * pa[m * RW];
+                        v10 += Cw[m] * pa[m * RW + 1];
+                        v01 += Cw[m] * pb[m * RW];
+                        v11 += Cw[m] * pb[m * RW + 1];
                     }
-                    val = (s * s * s) * a3 *
-                          rtm::bilinear<T>(fx, fy, tmax0(v00), tmax0(v01), tmax0(v10), tmax0(v11));
+                    /* state is alpha^-3 I: the (alpha/alpha_k)^3 factors cancel */
+                    val = (s * s * s) * rtm::bilinear<T>(fx, fy, tmax0(v00), tmax0(v01), tmax0(v10), tmax0(v11));
                 }
                 sv[k] = val;
                 const double vd = (double)val;
 #pragma unroll
-                for (int X = 0; X < 6; X++) acc[X] += w6[d * 6 + X] * vd;
+                for (int X = 0; X < 6; X++) acc[X] += dtab->w6[d][X] * vd;
             }
         } else {
 #pragma unroll
@@ -309,8 +382,8 @@ __global__ __launch_bounds__(256) void k_fused(KP<T> p)
                 } else {
                     /* reference arithmetic of StreamFlatFixed for this cell */
                     double xT = p.startx + gi * p.dx, yT = p.starty + j * p.dy;
-                    xT -= dtab->cx[d] * p.dtime;
-                    yT -= dtab->cy[d] * p.dtime;
+                    xT -= dtab->cx[d] * dtime;
+                    yT -= dtab->cy[d] * dtime;
                     double iT = (xT - p.startx) / p.dx, jT = (yT - p.starty) / p.dy;
                     double i0 = floor(iT), j0 = floor(jT);
                     oi = (int)i0 - gi;
@@ -320,37 +393,36 @@ __global__ __launch_bounds__(256) void k_fused(KP<T> p)
                     fx = (T)(iT - (double)(gi + oi));
                     fy = (T)(jT - (double)(j + oj));
                 }
-                const int sa = (j + oj + 3) % 3, sb = (j + oj + 4) % 3;
-                const int ca = lane + 1 + oi, cb = ca + 1;
-                T val = T(0);
-                if (active)
-                    val = rtm::bilinear<T>(fx, fy, sm.I[sa][d][ca], sm.I[sb][d][ca], sm.I[sa][d][cb], sm.I[sb][d][cb]);
+                const int ra = (j + oj + 3) % 3, rb = (j + oj + 4) % 3;
+                const int ca = lane + 1 + oi;
+                const T *pa = &sm.I[ra][d][ca];
+                const T *pb = &sm.I[rb][d][ca];
+                T val = rtm::bilinear<T>(fx, fy, pa[0], pb[0], pa[1], pb[1]);
+                val = active ? val : T(0);
                 sv[k] = val;
                 const double vd = (double)val;
 #pragma unroll
-                for (int X = 0; X < 6; X++) acc[X] += w6[d * 6 + X] * vd;
+                for (int X = 0; X < 6; X++) acc[X] += dtab->w6[d][X] * vd;
             }
         }
 #pragma unroll
         for (int X = 0; X < 6; X++) sm.X[wv][X][lane] = acc[X];
-        __syncthreads();
+        __syncthreads(); /* partial moments visible; every wave is done with slot (j-1)%3 and C */
+        if (more) {
+            stage_row(j + 2 <= p.ny - 1 ? j + 2 : p.ny - 1);
+            stage_coef(j + 1);
+        }
 
+        /* moments of the streamed intensities (true units: alpha^3 x state) */
+        const double a3 = CURVED ? 1.0 / (double)f_ia3 : 1.0;
         double mI[6];
 #pragma unroll
         for (int X = 0; X < 6; X++)
-            mI[X] = ((sm.X[0][X][lane] + sm.X[1][X][lane]) + sm.X[2][X][lane]) + sm.X[3][X][lane];
+            mI[X] = a3 * (((sm.X[0][X][lane] + sm.X[1][X][lane]) + sm.X[2][X][lane]) + sm.X[3][X][lane]);
 
         /* ---- collide ---- */
-        double eta = 0, k0 = 0, k1 = 0, kA = 0, u1 = 0, u2 = 0, adt = 0;
-        if (active) {
-            eta = (double)p.cell[CF_ETA * N + ij];
-            k0 = (double)p.cell[CF_K0 * N + ij];
-            k1 = (double)p.cell[CF_K1 * N + ij];
-            kA = (double)p.cell[CF_KA * N + ij];
-            u1 = (double)p.cell[CF_UX * N + ij];
-            u2 = (double)p.cell[CF_UY * N + ij];
-            adt = (double)p.cell[CF_ADT * N + ij];
-        }
+        const double eta = (double)f_eta, k0 = (double)f_k0, k1 = (double)f_k1, kA = (double)f_kA;
+        const double u1 = (double)f_ux, u2 = (double)f_uy, adt = (double)f_adt;
         const bool moving = active && (u1 != 0.0 || u2 != 0.0);
         double mOut[6];
         int cycles = 0;
@@ -383,97 +455,43 @@ __global__ __launch_bounds__(256) void k_fused(KP<T> p)
             }
 #pragma unroll
             for (int X = 0; X < 6; X++) mOut[X] = prev[X];
-            const T qT = (T)q, bT = (T)b, cxT = (T)(cc * fFx), cyT = (T)(cc * fFy);
+            /* Inew = J + (q-1) J + b + c n.F: the small factor q-1 carries the
+             * rounding, which keeps FP32 storage unbiased; b and c are scaled
+             * to the state's alpha^-3 units */
+            const double is = CURVED ? (double)f_ia3 : 1.0;
+            const T qm1 = (T)(q - 1.0), bT = (T)(b * is), cxT = (T)(cc * fFx * is), cyT = (T)(cc * fFy * is);
 #pragma unroll
             for (int k = 0; k < DW; k++) {
                 const int d = wv * DW + k;
                 const T nxd = (sizeof(T) == 4) ? (T)dtab->cxf[d] : (T)dtab->cx[d];
                 const T nyd = (sizeof(T) == 4) ? (T)dtab->cyf[d] : (T)dtab->cy[d];
-                sv[k] = qT * sv[k] + (bT + (cxT * nxd + cyT * nyd));
+                sv[k] = sv[k] + (qm1 * sv[k] + (bT + (cxT * nxd + cyT * nyd)));
             }
         } else {
-            /* moving fluid somewhere in this row: the reference's per-direction
-             * cycle (src/Radiation.cpp:661-722) with a moment exchange per
-             * cycle.  Every wavefront evaluates the same per-cell decisions, so
-             * the loop trip count is uniform across the workgroup. */
-            const double uu = u1 * u1 + u2 * u2;
-            const double lor = 1.0 / sqrt(1.0 - uu);
-            double prev[6];
+            if (CURVED) {
 #pragma unroll
-            for (int X = 0; X < 6; X++) prev[X] = mI[X];
-            double diff = 1;
-            T fin[DW];
-#pragma unroll
-            for (int k = 0; k < DW; k++) fin[k] = sv[k];
-            while (true) {
-                const bool iter = active && fabs(diff) > 1e-4 && cycles < 100;
-                if (!__any(iter)) break;
-                double part[6] = {0, 0, 0, 0, 0, 0};
-                if (iter) {
-                    cycles++;
-                    const double uF = u1 * prev[1] + u2 * prev[2];
-                    const double uxP = u1 * prev[3] + u2 * prev[4];
-                    const double uyP = u1 * prev[4] + u2 * prev[5];
-                    const double uuP = u1 * uxP + u2 * uyP;
-                    const double fE = lor * lor * (prev[0] - 2.0 * uF + uuP);
-                    const double fFx = lor * lor * (prev[1] - lor * prev[0] * u1 - uxP +
-                                                    u1 * lor / (1.0 + lor) * ((2.0 * lor + 1.0) * uF - lor * uuP));
-                    const double fFy = lor * lor * (prev[2] - lor * prev[0] * u2 - uyP +
-                                                    u2 * lor / (1.0 + lor) * ((2.0 * lor + 1.0) * uF - lor * uuP));
-#pragma unroll
-                    for (int k = 0; k < DW; k++) {
-                        const int d = wv * DW + k;
-                        const double n1 = dtab->cx[d], n2 = dtab->cy[d];
-                        const double un = u1 * n1 + u2 * n2;
-                        const double A = lor * (1.0 - un);
-                        const double fac = (1 - (lor * un) / (lor + 1)) * lor;
-                        const double nF = ((n1 - fac * u1) / A) * fFx + ((n2 - fac * u2) / A) * fFy;
-                        const double M = k0 * fE + 3.0 * k1 * nF;
-                        const double v = ((double)sv[k] + adt * (eta + M) / (A * A)) / (1.0 + adt * (kA + k0) * A);
-                        fin[k] = (T)v;
-#pragma unroll
-                        for (int X = 0; X < 6; X++) part[X] += w6[d * 6 + X] * v;
-                    }
-                }
-                __syncthreads();
-#pragma unroll
-                for (int X = 0; X < 6; X++) sm.X[wv][X][lane] = part[X];
-                __syncthreads();
-                if (iter) {
-                    double ss = 0;
-#pragma unroll
-                    for (int X = 0; X < 6; X++) {
-                        const double nw = ((sm.X[0][X][lane] + sm.X[1][X][lane]) + sm.X[2][X][lane]) + sm.X[3][X][lane];
-                        const double r = (prev[X] - nw) / nw;
-                        ss += r * r;
-                        prev[X] = nw;
-                    }
-                    diff = sqrt(ss);
-                }
+                for (int k = 0; k < DW; k++) sv[k] = (T)((double)sv[k] * a3);
             }
+            collide_moving<T, DW>(sv, mOut, cycles, mI, active, eta, k0, k1, kA, u1, u2, adt, wv, lane, dtab,
+                                  sm.X);
+            if (CURVED) {
 #pragma unroll
-            for (int X = 0; X < 6; X++) mOut[X] = prev[X];
-#pragma unroll
-            for (int k = 0; k < DW; k++) sv[k] = fin[k];
+                for (int k = 0; k < DW; k++) sv[k] = (T)((double)sv[k] * (double)f_ia3);
+            }
         }
 
         /* ---- write back ---- */
         if (active) {
 #pragma unroll
-            for (int k = 0; k < DW; k++) p.Iout[(long)(wv * DW + k) * N + ij] = sv[k];
+            for (int k = 0; k < DW; k++) (p.Iout + (size_t)(wv * DW + k) * N)[ij] = sv[k];
             if (wv == 0) {
                 it_sum += (unsigned long long)cycles;
                 it_max = it_max > (unsigned long long)cycles ? it_max : (unsigned long long)cycles;
                 if (p.want_moments) {
 #pragma unroll
-                    for (int X = 0; X < 6; X++) p.mom[(long)X * N + ij] = mOut[X];
+                    for (int X = 0; X < 6; X++) (p.mom + (size_t)X * N)[ij] = mOut[X];
                 }
             }
-        }
-        if (more) {
-            __syncthreads(); /* every wavefront is done with slot (j-1)%3 and the exchange buffer */
-            store_row(j + 2);
-            store_coef(j + 1);
         }
     }
 

@@ -70,6 +70,19 @@ __global__ void k_copy_fixed(const T *__restrict__ src, T *__restrict__ dst, con
     if (!(mask[ij] & MASK_ACTIVE)) dst[i] = src[i];
 }
 
+/* Curved streaming multiplies every gathered intensity by (alpha/alpha_k)^3
+ * (src/Radiation.cpp:497-505).  The fused kernel keeps the state as
+ * alpha^-3 I, so the factor becomes the destination's alpha^3 alone. */
+template <typename T>
+__global__ void k_scale_state(T *__restrict__ I, const T *__restrict__ ia3, int N, int D, int inverse)
+{
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * D) return;
+    const T s = ia3[i % N];
+    if (!(s > T(0)) || !(s < T(INFINITY))) return;
+    I[i] = inverse ? I[i] / s : I[i] * s;
+}
+
 /* Cell mask: active = interior and not InsideBH at the cell centre
  * (src/Radiation.cpp:644-649); near = the departure point may be excised. */
 __global__ void k_mask(uint8_t *mask, int nx, int ny, double startx, double starty, double dx, double dy,
@@ -643,6 +656,19 @@ int launch_copy_fixed_cells(rt_ctx *c, const void *src, void *dst)
         k_copy_fixed<float><<<nblk(n, 256), 256, 0, c->stream>>>((const float *)src, (float *)dst, c->d_mask, c->N, c->D);
     else
         k_copy_fixed<double><<<nblk(n, 256), 256, 0, c->stream>>>((const double *)src, (double *)dst, c->d_mask, c->N, c->D);
+    RT_HIP(c, hipGetLastError());
+    return RT_OK;
+}
+int launch_scale_state(rt_ctx *c, void *buf, bool inverse)
+{
+    size_t n = (size_t)c->N * c->D;
+    const size_t off = (size_t)CF_IA3 * c->N;
+    if (c->fp32)
+        k_scale_state<float><<<nblk(n, 256), 256, 0, c->stream>>>((float *)buf, (const float *)c->d_cell + off, c->N, c->D,
+                                                                 inverse ? 1 : 0);
+    else
+        k_scale_state<double><<<nblk(n, 256), 256, 0, c->stream>>>((double *)buf, (const double *)c->d_cell + off, c->N,
+                                                                   c->D, inverse ? 1 : 0);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }

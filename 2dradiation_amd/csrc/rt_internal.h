@@ -75,6 +75,8 @@ struct rt_ctx {
     bool has_I = false;
     void *d_I[2] = {nullptr, nullptr};
     int cur = 0;
+    /* curved fused stepping keeps the state as alpha^-3 I in d_I[cur] */
+    bool scaled = false;
     double *d_mom = nullptr;      /* [6][N] */
     int32_t *d_count = nullptr;   /* [N] per-method collide cycle counts */
     unsigned long long *d_stats = nullptr; /* [2] sum, max */
@@ -93,6 +95,9 @@ int rt_hip_check(rt_ctx *c, hipError_t e, const char *what);
 int launch_transpose_in(rt_ctx *c, const double *aos, void *soa);
 int launch_transpose_out(rt_ctx *c, const void *soa, double *aos);
 int launch_copy_fixed_cells(rt_ctx *c, const void *src, void *dst);
+/* state <- I alpha^-3 (inverse = false) or I <- state alpha^3 (inverse = true)
+ * for every cell with a finite positive lapse */
+int launch_scale_state(rt_ctx *c, void *buf, bool inverse);
 int launch_prepare_cells(rt_ctx *c);
 int launch_ref_stream(rt_ctx *c, const void *in, void *out);
 int launch_ref_moments(rt_ctx *c, const void *in);

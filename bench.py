@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB mebibytes (HBM counter calibration)")
     ap.add_argument("--ref-steps", type=int, default=200,
                     help="steps of the FP64 reference-order comparison leg (0 skips it)")
     args = ap.parse_args()
@@ -95,6 +97,16 @@ def main():
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(the domain-decomposition design is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.rand(n, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 * n}))
+        return
     rt = importlib.import_module("2dradiation_amd")
     rt.lib()
     spec = rt.workloads.WORKLOADS[WORKLOAD]
