@@ -55,6 +55,23 @@ size_t tsize(const rt_ctx *c) { return c->fp32 ? sizeof(float) : sizeof(double);
 /* Builds the per-direction constants of the fused kernel from the stencil and,
  * for the curved path, checks that the cubic tables repeat with period 10 fine
  * angles (true for the uniform stencils Stencil(n,0) builds). */
+/* Rounds five weights to multiples of 2^-23 (exact in FP32 for |w| < 2) and
+ * puts the remainder of target - sum on the largest one, so the FP32 weights
+ * sum to target exactly. */
+void exact_sum_f32(const double w[5], double target, float out[5])
+{
+    const double q = 0x1p-23;
+    long long k[5], sum = 0;
+    int big = 0;
+    for (int m = 0; m < 5; m++) {
+        k[m] = std::llround(w[m] / q);
+        sum += k[m];
+        if (std::fabs(w[m]) > std::fabs(w[big])) big = m;
+    }
+    k[big] += std::llround(target / q) - sum;
+    for (int m = 0; m < 5; m++) out[m] = (float)((double)k[m] * q);
+}
+
 int build_dir_tables(rt_ctx *c, const double *w, const double *phi, const double *cx, const double *cy,
                      const int32_t *Nc, const double *Wc)
 {
@@ -136,9 +153,18 @@ int build_dir_tables(rt_ctx *c, const double *w, const double *phi, const double
             const double a = k < 8 ? W0r[k] : 0.0, b = k < 8 ? W1r[k] : 0.0;
             h->wA[r][m] = a;
             h->wD[r][m] = b - a;
-            h->wAf[r][m] = (float)a;
-            h->wDf[r][m] = (float)(b - a);
         }
+        /* FP32 copies whose exact sums are 1 and 0, like the FP64 weights:
+         * every cell shares these rows, so a rounding error in their sum
+         * would act as one coherent gain on all rays */
+        exact_sum_f32(h->wA[r], 1.0, h->wAf[r]);
+        exact_sum_f32(h->wD[r], 0.0, h->wDf[r]);
+        for (int m = 0; m < 5; m++) {
+            h->wrowf[r][m] = h->wAf[r][m];
+            h->wrowf[r][5 + m] = h->wDf[r][m];
+        }
+        std::memcpy(&h->wrowf[r][10], &h->woff[r], sizeof(float));
+        h->wrowf[r][11] = 0.0f;
     }
     c->fused_tables_ok = ok;
     hipError_t e = hipMemcpyAsync(c->d_dirt, h, sizeof *h, hipMemcpyHostToDevice, c->stream);
@@ -228,7 +254,10 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
         (r = dalloc(c, &c->d_cx, D * sizeof(double))) || (r = dalloc(c, &c->d_cy, D * sizeof(double))) ||
         (r = dalloc(c, &c->d_basis, D * d.nCoeff * sizeof(double))))
         return fail(r);
-    if (d.streaming == RT_CURVED_FIXED && (r = dalloc(c, &c->d_coef, 5 * (size_t)d.nCoeff * N * sizeof(double)))) return fail(r);
+    if (d.streaming == RT_CURVED_FIXED && ((r = dalloc(c, &c->d_coef, 5 * (size_t)d.nCoeff * N * sizeof(double))) ||
+                                           (r = dalloc(c, &c->d_excl, N * sizeof(unsigned long long)))))
+        return fail(r);
+    if ((r = dalloc(c, &c->d_red, sizeof(unsigned long long)))) return fail(r);
     /* zeroed state; flat geometry until rt_set_geometry (alpha = 1, tetrad = I) */
     if (hipMemsetAsync(c->d_I[0], 0, N * D * tsize(c), c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_I[1], 0, N * D * tsize(c), c->stream) != hipSuccess ||
@@ -252,7 +281,7 @@ void rt_destroy(rt_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_w); dfree(c->d_phi); dfree(c->d_cx); dfree(c->d_cy); dfree(c->d_Nc); dfree(c->d_Wc);
     dfree(c->d_basis); dfree(c->d_dirt); dfree(c->d_alpha); dfree(c->d_t11); dfree(c->d_t12);
-    dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_mask); dfree(c->d_f64);
+    dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_excl); dfree(c->d_red); dfree(c->d_mask); dfree(c->d_f64);
     dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_mom); dfree(c->d_count);
     dfree(c->d_stats);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -3,10 +3,11 @@
  *
  * One workgroup = 4 wavefronts walks a 64-cell-wide column strip of the grid
  * down a chunk of rows.  Three input rows (all directions) live in an LDS ring
- * [slot][d][66] (64 cells plus one halo cell each side); the row two ahead is
- * prefetched into registers with coalesced SoA loads while the current row is
- * computed, then written into the slot that just died.  Lane = cell column,
- * wavefront w = direction block [w*D/4, (w+1)*D/4).
+ * [slot][direction][66] (64 cells plus one halo cell each side, directions
+ * 0..3 repeated after D-1); the row two ahead is loaded straight from HBM
+ * into the slot that just died (global_load_lds, no register staging) while
+ * the current row is computed.  Lane = cell column, wavefront w = direction
+ * block [w*D/4, (w+1)*D/4).
  *
  * Stream (reference src/Radiation.cpp:380-414 flat, :454-512 curved):
  *   flat:   bilinear of the same direction at the departure point x - c dt.
@@ -14,9 +15,9 @@
  *           at phi_d with the per-direction basis from scalar registers; the
  *           departure velocity angle (Hastings atan2) picks the cubic angular
  *           window; the lerp between the two cubic stencils is folded into 5
- *           combined taps; each of the 4 bilinear corners is clamped >= 0; the
- *           (alpha/alpha_k)^3 factor is applied as alpha^3 * (I_k alpha_k^-3)
- *           with the alpha_k^-3 scaling done once while staging into LDS.
+ *           combined taps; each of the 4 bilinear corners is clamped >= 0.
+ *           The state is held as alpha^-3 I while stepping, so the reference's
+ *           (alpha/alpha_k)^3 factor needs no per-corner work.
  * Moments (src/Radiation.cpp:239-269): per wavefront partial sums in FP64,
  *   combined across the 4 wavefronts through LDS in a fixed order.
  * Collide (src/Radiation.cpp:641-736): for cells with zero fluid velocity
@@ -28,6 +29,11 @@
  *   reference's stopping rule; the intensities are formed once at the end.
  *   Rows containing a moving cell run the reference's per-direction cycle
  *   with a workgroup-wide moment exchange per cycle.
+ *
+ * FP32-mixed accuracy: tables that every cell shares (angular weights, the
+ * Fourier basis) are the one FP32 error that does not average out along a
+ * ray, so the angular weight rows sum exactly to 1 and 0 in FP32 and the
+ * ray factor s enters as s - 1.
  */
 #include "rt_internal.h"
 
@@ -40,6 +46,7 @@ struct KP {
     const T *cell;              /* [RT_CELL_FIELDS][N] */
     const uint8_t *mask;
     const double *coef;         /* [25][N] FP64 */
+    const unsigned long long *excl; /* [N] excised-departure bits */
     double *mom;                /* [6][N] */
     unsigned long long *stats;  /* [2] */
     const DirTables *dt;
@@ -79,11 +86,11 @@ typedef const __attribute__((address_space(4))) DirTables CDirTables;
 template <typename T, int D, bool CURVED>
 struct Smem {
     static constexpr int RW = 66;
+    static constexpr int NC = CURVED ? 26 : 1; /* 25 Fourier coefficients + exclusion bits */
     T I[3][D + 4][RW];
-    double C[CURVED ? 25 : 1][64];
+    double C[NC][64];
     double X[RT_FUSED_WAVES][6][64];
-    T wA[10][5], wD[10][5];
-    int woff[10];
+    float wrow[10][12] __attribute__((aligned(16)));
 };
 
 /* Copies n dwords from global src to LDS dst with one dword per lane per
@@ -106,10 +113,15 @@ __device__ inline T angle_index_bf(T vx, T vy, T u, int quarter)
     const bool swap = !(fabsf((float)vx) > fabsf((float)vy));
     const T num = swap ? vx : vy, den = swap ? vy : vx;
     const T p = rtm::hastings_atan(fast_div(num, den));
+    /* octant offset in quarters: |vx| > |vy|: 0 (or 4 when p < 0) for vx > 0,
+     * 2 for vx < 0; otherwise 1 for vy > 0, 3 for vy < 0 (and the
+     * polynomial enters with a minus sign) */
+    const int px = (int)(vx > T(0)), pneg = (int)(p < T(0)), py = (int)(vy > T(0));
+    const int kx = 2 - 2 * px + 4 * (px & pneg);
+    const int ky = 3 - 2 * py;
+    const int k = swap ? ky : kx;
     const T up = u * p;
-    const T off_a = vx > T(0) ? (p < T(0) ? T(4 * quarter) : T(0)) : T(2 * quarter);
-    const T off_b = vy > T(0) ? T(quarter) : T(3 * quarter);
-    T dd = swap ? off_b - up : off_a + up;
+    const T dd = T(k * quarter) + (swap ? -up : up);
     return (vx == T(0) && vy == T(0)) ? T(0) : dd;
 }
 
@@ -196,6 +208,8 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
 {
     constexpr int DW = D / RT_FUSED_WAVES;
     constexpr int RW = Smem<T, D, CURVED>::RW;
+    constexpr int NC = Smem<T, D, CURVED>::NC;
+    static_assert(DW <= 32, "exclusion bits of one wavefront must fit 32 bits");
     __shared__ Smem<T, D, CURVED> sm;
 
     const int lane = threadIdx.x & 63;
@@ -220,22 +234,11 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
     if (rowA >= rowB) return;
     const int ncols = min(RW, nx - col0);       /* valid LDS columns */
 
-    if (CURVED) {
-        if (threadIdx.x < 50) {
-            int r = threadIdx.x / 5, m = threadIdx.x % 5;
-            if constexpr (sizeof(T) == 4) {
-                sm.wA[r][m] = dtab0->wAf[r][m];
-                sm.wD[r][m] = dtab0->wDf[r][m];
-            } else {
-                sm.wA[r][m] = dtab0->wA[r][m];
-                sm.wD[r][m] = dtab0->wD[r][m];
-            }
-        }
-        if (threadIdx.x < 10) sm.woff[threadIdx.x] = dtab0->woff[threadIdx.x];
+    if constexpr (CURVED && sizeof(T) == 4) {
+        if (threadIdx.x < 120) (&sm.wrow[0][0])[threadIdx.x] = (&dtab0->wrowf[0][0])[threadIdx.x];
     }
 
     /* ---- asynchronous staging (direct global -> LDS) ---- */
-    constexpr int RDW = (int)(RW * sizeof(T) / 4); /* dwords per LDS row */
     auto stage_row = [&](int row) {
         const int slot = row % 3;
         const int ndw = (int)(ncols * sizeof(T) / 4);
@@ -244,14 +247,15 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
             const T *src = p.Iin + (size_t)d * N + (size_t)row * nx + col0;
             lds_dma_dwords(src, &sm.I[slot][r][0], ndw, lane);
         }
-        (void)RDW;
     };
     auto stage_coef = [&](int row) {
-        if (!CURVED) return;
-        const int n = (int)(min(64, nx - 1 - (col0 + 1)) * 2); /* interior cells of the strip */
-        for (int e = wv; e < 25; e += RT_FUSED_WAVES) {
-            const double *src = p.coef + (size_t)e * N + (size_t)row * nx + col0 + 1;
-            lds_dma_dwords(src, &sm.C[e][0], n, lane);
+        if constexpr (CURVED) {
+            const int n = (int)(min(64, nx - 1 - (col0 + 1)) * 2); /* interior cells of the strip */
+            const size_t off = (size_t)row * nx + col0 + 1;
+            for (int e = wv; e < NC; e += RT_FUSED_WAVES) {
+                const void *src = e < 25 ? (const void *)(p.coef + (size_t)e * N + off) : (const void *)(p.excl + off);
+                lds_dma_dwords(src, &sm.C[e][0], n, lane);
+            }
         }
     };
 
@@ -261,11 +265,14 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
 
     unsigned long long it_sum = 0, it_max = 0;
     const double dtime = p.dtime;
+    int sl = (rowA - 1) % 3; /* LDS slot of row j-1 */
 
     for (int j = rowA; j < rowB; j++) {
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads(); /* slots j-1..j+1 and the coefficients of row j are in LDS */
         const bool more = (j + 1 < rowB);
+        const int slm = sl, sl0 = sl == 2 ? 0 : sl + 1, slp = sl0 == 2 ? 0 : sl0 + 1;
+        sl = sl0;
 
         const unsigned ij = (unsigned)j * (unsigned)nx + (unsigned)gi;
         const uint8_t mk = inside ? p.mask[ij] : 0;
@@ -287,30 +294,50 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
         CDirTables *dtab = dtab0;
         asm volatile("" : "+s"(dtab));
 
+        /* this lane's column in the three row slots */
+        const T *colm = &sm.I[slm][0][lane];
+        const T *col0p = &sm.I[sl0][0][lane];
+        const T *colp = &sm.I[slp][0][lane];
+
         /* ---- stream ---- */
         T sv[DW];
         double acc[6] = {0, 0, 0, 0, 0, 0};
-        if (CURVED) {
-            /* geodesic Fourier coefficients of this cell; X and Y relative to
-             * the cell centre in cell units */
+        if constexpr (CURVED) {
+            /* geodesic Fourier coefficients of this cell: S as S - 1, X and Y
+             * relative to the cell centre in cell units */
             T cS[5], cX[5], cY[5], cU[5], cV[5];
             {
                 const double xc = p.startx + gi * p.dx, yc = p.starty + j * p.dy;
                 const double idx = 1.0 / p.dx, idy = 1.0 / p.dy;
 #pragma unroll
                 for (int k = 0; k < 5; k++) {
-                    cS[k] = (T)sm.C[k][lane];
+                    cS[k] = (T)(k == 0 ? sm.C[0][lane] - 1.0 : sm.C[k][lane]);
                     cX[k] = (T)((k == 0 ? sm.C[5][lane] - xc : sm.C[5 + k][lane]) * idx);
                     cY[k] = (T)((k == 0 ? sm.C[10][lane] - yc : sm.C[10 + k][lane]) * idy);
                     cU[k] = (T)sm.C[15 + k][lane];
                     cV[k] = (T)sm.C[20 + k][lane];
                 }
             }
-            const bool near = (mk & MASK_NEAR_BH) != 0;
+            /* directions of this wavefront whose departure point is excised */
+            const uint32_t exw = (uint32_t)(__double_as_longlong(sm.C[25][lane]) >> (wv * DW));
             const T angu = (sizeof(T) == 4) ? (T)dtab->ang_uf : (T)dtab->ang_u;
             const int quarter = dtab->quarter, nGrid = dtab->nGrid;
-#pragma unroll
-            for (int k = 0; k < DW; k++) {
+            const T ddmax = T(nGrid) - T(1e-8);
+            /* Software pipeline over the directions of this wavefront: the
+             * LDS taps of direction k+1 are in flight while direction k is
+             * combined.  prep() computes the departure point, angular window
+             * and weights and issues the 20 tap loads; fin() clamps,
+             * interpolates and applies the ray factor.  Everything is
+             * computed for every lane (no branches); lanes that do not gather
+             * select 0.  The clamps keep every LDS index in range for any
+             * input, NaN included. */
+            struct Prep {
+                T v[5][4];
+                T Cw[5];
+                T fx, fy, s;
+                bool go;
+            };
+            auto prep = [&](int k, Prep &P) {
                 const int d = wv * DW + k;
                 T B[5];
 #pragma unroll
@@ -324,49 +351,86 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
                     vx += cU[q] * B[q];
                     vy += cV[q] * B[q];
                 }
-                bool go = active;
-                if (near) {
-                    const double xd = p.startx + gi * p.dx + (double)di * p.dx;
-                    const double yd = p.starty + j * p.dy + (double)dj * p.dy;
-                    if (xd * xd + yd * yd <= p.r2) go = false;
-                }
-                T val = T(0);
-                if (go) {
-                    T fi = floor(di), fj = floor(dj);
-                    fi = fi < T(-1) ? T(-1) : (fi > T(0) ? T(0) : fi);
-                    fj = fj < T(-1) ? T(-1) : (fj > T(0) ? T(0) : fj);
-                    const T fx = di - fi, fy = dj - fj;
-                    T dd = angle_index_bf<T>(vx, vy, angu, quarter);
-                    dd = dd < T(1e-8) ? T(1e-8) : dd;
-                    dd = dd > T(nGrid) - T(1e-8) ? T(nGrid) - T(1e-8) : dd;
-                    int g0 = (int)dd;
-                    const T t = dd - T(g0);
-                    g0 = (unsigned)g0 < (unsigned)nGrid ? g0 : 0; /* NaN-safe LDS index */
-                    const int r = g0 % 10;
-                    int s0 = g0 / 10 + sm.woff[r];
-                    s0 = s0 < 0 ? s0 + D : (s0 >= D ? s0 - D : s0);
-                    T Cw[5];
+                P.s = s;
+                P.go = active && !((exw >> k) & 1u);
+                const T fi = fmin(fmax(floor(di), T(-1)), T(0));
+                const T fj = fmin(fmax(floor(dj), T(-1)), T(0));
+                P.fx = di - fi;
+                P.fy = dj - fj;
+                T dd = angle_index_bf<T>(vx, vy, angu, quarter);
+                dd = fmin(fmax(dd, T(1e-8)), ddmax);
+                int g0 = (int)dd;
+                const T t = dd - T(g0);
+                g0 = (unsigned)g0 < (unsigned)nGrid ? g0 : 0; /* dd == nGrid after rounding is angle 0 */
+                const int q10 = (int)(__umul24((unsigned)g0, 6554u) >> 16); /* g0 / 10 for g0 < 16384 */
+                const int r = g0 - 10 * q10;
+                int woff;
+                if constexpr (sizeof(T) == 4) {
+                    const float4 *wr = (const float4 *)((const char *)&sm.wrow[0][0] + __umul24((unsigned)r, 48u));
+                    const float4 w0 = wr[0], w1 = wr[1], w2 = wr[2];
+                    P.Cw[0] = w0.x + t * w1.y;
+                    P.Cw[1] = w0.y + t * w1.z;
+                    P.Cw[2] = w0.z + t * w1.w;
+                    P.Cw[3] = w0.w + t * w2.x;
+                    P.Cw[4] = w1.x + t * w2.y;
+                    woff = __float_as_int(w2.z);
+                } else {
+                    /* FP64 rows come through the vector L1 (faster than LDS
+                     * for 8-byte elements) */
 #pragma unroll
-                    for (int m = 0; m < 5; m++) Cw[m] = sm.wA[r][m] + t * sm.wD[r][m];
-                    const int ra = (j + (int)fj + 3) % 3, rb = (j + (int)fj + 4) % 3;
-                    const int ca = lane + 1 + (int)fi;
-                    const T *pa = &sm.I[ra][s0][ca];
-                    const T *pb = &sm.I[rb][s0][ca];
-                    T v00 = 0, v01 = 0, v10 = 0, v11 = 0;
-#pragma unroll
-                    for (int m = 0; m < 5; m++) {
-                        v00 += Cw[m] * pa[m * RW];
-                        v10 += Cw[m] * pa[m * RW + 1];
-                        v01 += Cw[m] * pb[m * RW];
-                        v11 += Cw[m] * pb[m * RW + 1];
-                    }
-                    /* state is alpha^-3 I: the (alpha/alpha_k)^3 factors cancel */
-                    val = (s * s * s) * rtm::bilinear<T>(fx, fy, tmax0(v00), tmax0(v01), tmax0(v10), tmax0(v11));
+                    for (int m = 0; m < 5; m++) P.Cw[m] = dtab0->wA[r][m] + t * dtab0->wD[r][m];
+                    woff = dtab0->woff[r];
                 }
+                int s0 = q10 + woff;
+                s0 = s0 < 0 ? s0 + D : (s0 >= D ? s0 - D : s0);
+                const bool lo_j = fj < T(0);
+                const T *ra = lo_j ? colm : col0p;
+                const T *rb = lo_j ? col0p : colp;
+                const int off = (int)__umul24((unsigned)s0, (unsigned)RW) + (fi < T(0) ? 0 : 1);
+                const T *pa = ra + off;
+                const T *pb = rb + off;
+#pragma unroll
+                for (int m = 0; m < 5; m++) {
+                    P.v[m][0] = pa[m * RW];
+                    P.v[m][1] = pa[m * RW + 1];
+                    P.v[m][2] = pb[m * RW];
+                    P.v[m][3] = pb[m * RW + 1];
+                }
+            };
+            auto fin = [&](int k, const Prep &P) {
+                const int d = wv * DW + k;
+                T v00 = 0, v10 = 0, v01 = 0, v11 = 0;
+#pragma unroll
+                for (int m = 0; m < 5; m++) {
+                    v00 += P.Cw[m] * P.v[m][0];
+                    v10 += P.Cw[m] * P.v[m][1];
+                    v01 += P.Cw[m] * P.v[m][2];
+                    v11 += P.Cw[m] * P.v[m][3];
+                }
+                /* state is alpha^-3 I: the (alpha/alpha_k)^3 factors cancel.  s is
+                 * carried as s - 1, so s^3 x = x + (s^3 - 1) x rounds once, at the end */
+                const T bil = rtm::bilinear<T>(P.fx, P.fy, tmax0(v00), tmax0(v01), tmax0(v10), tmax0(v11));
+                const T s3m1 = P.s * (T(3) + P.s * (T(3) + P.s));
+                const T val = P.go ? bil + bil * s3m1 : T(0);
                 sv[k] = val;
+                /* materialise this direction's value here: otherwise the
+                 * compiler sinks every gather to the end of the row and
+                 * spills the taps of all directions */
+                asm volatile("" ::"v"(val));
                 const double vd = (double)val;
 #pragma unroll
                 for (int X = 0; X < 6; X++) acc[X] += dtab->w6[d][X] * vd;
+            };
+            Prep P0, P1;
+            prep(0, P0);
+#pragma unroll
+            for (int k = 0; k < DW; k += 2) {
+                if (k + 1 < DW) prep(k + 1, P1);
+                fin(k, P0);
+                if (k + 1 < DW) {
+                    if (k + 2 < DW) prep(k + 2, P0);
+                    fin(k + 1, P1);
+                }
             }
         } else {
 #pragma unroll
@@ -393,10 +457,11 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
                     fx = (T)(iT - (double)(gi + oi));
                     fy = (T)(jT - (double)(j + oj));
                 }
-                const int ra = (j + oj + 3) % 3, rb = (j + oj + 4) % 3;
-                const int ca = lane + 1 + oi;
-                const T *pa = &sm.I[ra][d][ca];
-                const T *pb = &sm.I[rb][d][ca];
+                const T *ra = oj < 0 ? colm : col0p;
+                const T *rb = oj < 0 ? col0p : colp;
+                const int off = d * RW + 1 + oi;
+                const T *pa = ra + off;
+                const T *pb = rb + off;
                 T val = rtm::bilinear<T>(fx, fy, pa[0], pb[0], pa[1], pb[1]);
                 val = active ? val : T(0);
                 sv[k] = val;
@@ -409,7 +474,7 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
         for (int X = 0; X < 6; X++) sm.X[wv][X][lane] = acc[X];
         __syncthreads(); /* partial moments visible; every wave is done with slot (j-1)%3 and C */
         if (more) {
-            stage_row(j + 2 <= p.ny - 1 ? j + 2 : p.ny - 1);
+            stage_row(j + 2);
             stage_coef(j + 1);
         }
 
@@ -434,23 +499,27 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
 #pragma unroll
             for (int X = 0; X < 6; X++) prev[X] = mI[X];
             double b = 0, fFx = 0, fFy = 0;
-            double diff = 1;
+            double ss = 1;
             if (active) {
-                while (fabs(diff) > 1e-4 && cycles < 100) {
+                /* stop when sqrt(sum ((prev - new) / new)^2) <= 1e-4; the
+                 * quotients use a refined reciprocal, and new == 0 keeps the
+                 * IEEE quotient (inf, or NaN for 0/0, which stops the loop) */
+                while (ss > 1e-8 && cycles < 100) {
                     cycles++;
                     b = adt * q * (eta + k0 * prev[0]);
                     fFx = prev[1];
                     fFy = prev[2];
-                    double nw[6], ss = 0;
+                    double acc2 = 0;
 #pragma unroll
                     for (int X = 0; X < 6; X++) {
-                        nw[X] = q * mI[X] + b * dtab->S[0][X] + cc * (fFx * dtab->S[1][X] + fFy * dtab->S[2][X]);
-                        const double r = (prev[X] - nw[X]) / nw[X];
-                        ss += r * r;
+                        const double nw = q * mI[X] + b * dtab->S[0][X] + cc * (fFx * dtab->S[1][X] + fFy * dtab->S[2][X]);
+                        const double r0 = __builtin_amdgcn_rcp(nw);
+                        const double r1 = fma(r0, fma(-nw, r0, 1.0), r0);
+                        const double r = (prev[X] - nw) * (nw == 0.0 ? r0 : r1);
+                        acc2 += r * r;
+                        prev[X] = nw;
                     }
-                    diff = sqrt(ss);
-#pragma unroll
-                    for (int X = 0; X < 6; X++) prev[X] = nw[X];
+                    ss = acc2;
                 }
             }
 #pragma unroll
@@ -518,6 +587,7 @@ int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments)
     p.cell = (const T *)c->d_cell;
     p.mask = c->d_mask;
     p.coef = c->d_coef;
+    p.excl = c->d_excl;
     p.mom = c->d_mom;
     p.stats = c->d_stats;
     p.dt = c->d_dirt;

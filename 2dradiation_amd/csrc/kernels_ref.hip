@@ -590,11 +590,13 @@ __global__ __launch_bounds__(128) void k_geodesic_tables(const double *__restric
  * kernel's one-cell halo covers every bilinear corner). */
 __global__ void k_departure_bound(const double *__restrict__ coef, const double *__restrict__ basis,
                                   const uint8_t *__restrict__ mask, int nx, int ny, int D, int nF, double startx,
-                                  double starty, double dx, double dy, double r2, unsigned long long *out)
+                                  double starty, double dx, double dy, double r2, unsigned long long *out,
+                                  unsigned long long *excl)
 {
     int ij = blockIdx.x * blockDim.x + threadIdx.x;
     const long N = (long)nx * ny;
     double m = 0;
+    unsigned long long bits = 0;
     if (ij < N && (mask[ij] & MASK_ACTIVE)) {
         int i = ij % nx, j = ij / nx;
         double xc = startx + i * dx, yc = starty + j * dy;
@@ -604,12 +606,16 @@ __global__ void k_departure_bound(const double *__restrict__ coef, const double 
                 xT += coef[((long)1 * nF + k) * N + ij] * basis[d * nF + k];
                 yT += coef[((long)2 * nF + k) * N + ij] * basis[d * nF + k];
             }
-            if (r2 >= 0.0 && xT * xT + yT * yT <= r2) continue; /* excised departure: not gathered */
+            if (r2 >= 0.0 && xT * xT + yT * yT <= r2) { /* excised departure: not gathered */
+                if (d < 64) bits |= 1ull << d;
+                continue;
+            }
             double a = fabs(xT - xc) / dx, b = fabs(yT - yc) / dy;
             if (!(a == a) || !(b == b)) { m = 1e30; break; }
             m = fmax(m, fmax(a, b));
         }
     }
+    if (ij < N) excl[ij] = bits;
     /* non-negative doubles order like their bit patterns */
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
@@ -772,10 +778,11 @@ int launch_geodesic_tables(rt_ctx *c, const double *d_metric, double tol)
 int launch_departure_bound(rt_ctx *c, double *max_disp)
 {
     const rt_desc &d = c->desc;
-    unsigned long long *tmp = c->d_stats; /* reuse: caller resets stats after */
+    unsigned long long *tmp = c->d_red;
     RT_HIP(c, hipMemsetAsync(tmp, 0, sizeof(unsigned long long), c->stream));
     k_departure_bound<<<nblk(c->N, 256), 256, 0, c->stream>>>(c->d_coef, c->d_basis, c->d_mask, d.nx, d.ny, c->D, c->nCoeff,
-                                                              d.startx, d.starty, d.dx, d.dy, d.excision_r2, tmp);
+                                                              d.startx, d.starty, d.dx, d.dy, d.excision_r2, tmp,
+                                                              c->d_excl);
     RT_HIP(c, hipGetLastError());
     unsigned long long bits = 0;
     RT_HIP(c, hipMemcpyAsync(&bits, tmp, sizeof bits, hipMemcpyDeviceToHost, c->stream));
@@ -783,6 +790,5 @@ int launch_departure_bound(rt_ctx *c, double *max_disp)
     double m;
     memcpy(&m, &bits, sizeof m);
     *max_disp = m;
-    RT_HIP(c, hipMemsetAsync(tmp, 0, 2 * sizeof(unsigned long long), c->stream));
     return RT_OK;
 }

@@ -28,6 +28,8 @@ struct DirTables {
     int woff[10];        /* window start offset per fine-angle residue */
     double wA[10][5], wD[10][5];   /* combined 5-tap angular weights: C = A + t*D */
     float wAf[10][5], wDf[10][5];
+    /* FP32 row per residue for 16-byte LDS reads: wA[5], wD[5], woff (int bits), 0 */
+    float wrowf[10][12];
     double ang_u;        /* nGrid / (2 pi) */
     float ang_uf;
     int quarter;         /* nGrid / 4 */
@@ -64,6 +66,9 @@ struct rt_ctx {
     double *d_coef = nullptr;     /* SoA [5 series][nCoeff][N], FP64 */
     uint8_t *d_mask = nullptr;
     double max_disp = 0;          /* max |departure - cell| in cells (curved tables) */
+    /* bit d set: the departure point of direction d (< 64) is excised */
+    unsigned long long *d_excl = nullptr;
+    unsigned long long *d_red = nullptr; /* one-word reduction target */
 
     /* fields */
     bool has_fields = false;
