@@ -1,0 +1,100 @@
+// Radiation transport driver with the reference's public interface
+// (reference: src/Radiation.h, src/Radiation.cpp).  Public members and
+// methods keep the reference's names so user programs compile unchanged; the
+// work runs on the GPU through the C ABI in include/rt2d.h.
+//
+// State model: the host buffers (I, E..Pyy, the fluid fields, ...) are the
+// user-visible state, as in the reference.  The single-step methods
+// (StreamFlatFixed, ComputeMomentsIF, Collide, ...) upload what they read and
+// download what they write, so they can be mixed with host code freely.
+// RunSimulation uploads once, keeps the state on the device for the whole
+// loop and downloads I and the moments at the end.
+//
+// Not part of this library (the methods exit with a message, like the
+// reference's ExitOnError): adaptive and geodesic streaming, initial data
+// given as moments (the flux-to-sigma lookup tables), lab-frame moments and
+// the CSV frame output.
+#ifndef RT2D_RADIATION_H
+#define RT2D_RADIATION_H
+#include "Config.hh"
+#include "Metric.h"
+#include "Stencil.h"
+
+struct rt_ctx;
+
+class Radiation {
+  private:
+    static constexpr int HALO = 1;
+    static constexpr double MIN_FLUX_NORM = 1e-16;
+    static constexpr double MIN_ENERGY_DENSITY = 1e-16;
+    static constexpr double LAMBDA_ITTERATION_TOLERENCE = 1e-4;
+    static constexpr int MAX_LAMBDA_ITERATIONS = 100;
+
+    rt_ctx *ctx = nullptr;
+    bool tablesOnDevice = false;
+    void Check(int status) const;
+    void UploadFields();
+    void UploadI();
+    void DownloadI();
+    void DownloadMoments();
+    void RequireSupportedStreaming() const;
+
+  public:
+    Grid &grid;
+    Metric &metric;
+    Stencil &stencil;
+    Stencil &streamingStencil;
+    Config config;
+    double sigmaOverwrite = -1;
+
+    bool *isInitialGridPoint;
+    DoubleBuffer initialI;
+    DoubleBuffer initialFluxAngle_IF;
+    DoubleBuffer initialE_LF, initialFx_LF, initialFy_LF;
+    DoubleBuffer initialPxx_LF, initialPxy_LF, initialPyy_LF;
+
+    DoubleBuffer rotationAngle, rotationAngleNew;
+
+    DoubleBuffer E, Fx, Fy, Pxx, Pxy, Pyy;
+    DoubleBuffer E_LF, Fx_LF, Fy_LF, Pxx_LF, Pxy_LF, Pyy_LF, F_LF;
+
+    DoubleBuffer kappa0, kappa1, kappaA, eta, ux, uy;
+
+    DoubleBuffer I, Inew;
+
+    DoubleBuffer coefficientsS, coefficientsX, coefficientsY, coefficientsCx, coefficientsCy;
+
+    IntBuffer itterationCount;
+    int maxItterationCount = 0;
+    double averageItterationCount = 0;
+
+    Radiation() = delete;
+    Radiation(Metric &metric, Stencil &stencil, Stencil &streamingStencil, Config config);
+    Radiation(const Radiation &) = delete;
+    Radiation &operator=(const Radiation &) = delete;
+    ~Radiation();
+
+    size_t Index(size_t ij, size_t d) const { return d + ij * stencil.nDir; }
+    size_t Index(size_t i, size_t j, size_t d) const { return d + grid.Index(i, j) * stencil.nDir; }
+    size_t HarmonicIndex(size_t f, size_t ij) const { return f + ij * streamingStencil.nCoefficients; }
+
+    void LoadInitialData();
+    void UpdateFourierCoefficients();
+    void ComputeMomentsIF();
+    void ComputeMomentsLF();
+
+    void StreamFlatFixed();
+    void StreamFlatAdaptive();
+    void StreamCurvedFixed();
+    void StreamCurvedAdaptive();
+    void StreamGeodesicFixed();
+
+    void Collide();
+
+    void RunSimulation();
+
+    // The device context, for callers that want the C ABI directly (e.g. to
+    // time steps); owned by this object.
+    rt_ctx *Context() const { return ctx; }
+};
+#endif

@@ -263,6 +263,7 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
         hipMemsetAsync(c->d_I[1], 0, N * D * tsize(c), c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_mom, 0, 6 * N * sizeof(double), c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_stats, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_count, 0, N * sizeof(int32_t), c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_f64, 0, 6 * N * sizeof(double), c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_t12, 0, N * sizeof(double), c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_t21, 0, N * sizeof(double), c->stream) != hipSuccess)
@@ -323,10 +324,12 @@ int rt_set_stencil(rt_ctx *c, const double *w, const double *phi, const double *
         if (c->nGrid != nGrid) {
             dfree(c->d_Nc);
             dfree(c->d_Wc);
+            c->d_Nc = nullptr;
+            c->d_Wc = nullptr;
         }
         c->nGrid = nGrid;
-        TRY(dalloc(c, &c->d_Nc, 4 * (size_t)nGrid * sizeof(int32_t)));
-        TRY(dalloc(c, &c->d_Wc, 4 * (size_t)nGrid * sizeof(double)));
+        if (!c->d_Nc) TRY(dalloc(c, &c->d_Nc, 4 * (size_t)nGrid * sizeof(int32_t)));
+        if (!c->d_Wc) TRY(dalloc(c, &c->d_Wc, 4 * (size_t)nGrid * sizeof(double)));
         RT_HIP(c, hipMemcpyAsync(c->d_Nc, Nc, 4 * (size_t)nGrid * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
         RT_HIP(c, hipMemcpyAsync(c->d_Wc, Wc, 4 * (size_t)nGrid * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
@@ -539,6 +542,15 @@ int rt_iteration_stats(rt_ctx *c, int32_t *maxCount, double *sumCount)
     RT_HIP(c, hipStreamSynchronize(c->stream));
     if (maxCount) *maxCount = (int32_t)s[1];
     if (sumCount) *sumCount = (double)s[0];
+    return RT_OK;
+}
+
+int rt_get_iteration_counts(rt_ctx *c, int32_t *count)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    NEED(c, count);
+    RT_HIP(c, hipMemcpyAsync(count, c->d_count, c->N * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
     return RT_OK;
 }
 

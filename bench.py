@@ -33,6 +33,18 @@ def algorithmic_bytes_per_update(nDir, b_i, b_f, curved=True, nCoeff=5):
     return 2 * b_i + (6 * b_f + b_geo) / nDir
 
 
+def profiled_traffic_bytes():
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    summary (FETCH_SIZE and WRITE_SIZE runs scaled by a plain-copy
+    calibration), or None if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_fused_f32_d64_schwarzschild_1024sq.md")
+    if not os.path.exists(path):
+        return None
+    import re
+    m = re.search(r"= \*\*([0-9.]+) MB\*\*", open(path).read())
+    return float(m.group(1)) * 1e6 if m else None
+
+
 def rel_l2(a, b):
     n = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
@@ -147,9 +159,13 @@ def main():
     B32 = algorithmic_bytes_per_update(spec["nDir"], 4, 4)
     per_launch_ms = ms32 / K
     achieved = B32 * upd_per_step / (per_launch_ms * 1e-3)
+    traffic = profiled_traffic_bytes()
     result["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK, "traffic": None,
+                          "frac": achieved / HBM_PEAK, "traffic": traffic,
+                          "traffic_over_algorithmic": traffic / (B32 * upd_per_step) if traffic else None,
                           "bytes_per_update": B32, "kernel": "k_fused<float,64,curved>"}
+    # the reference's own figure: stored cells x steps / s (src/Radiation.cpp:880)
+    result["mlups_reference_convention"] = gs.N * K / (ms32 * 1e-3) / 1e6
 
     if not args.no_fp64:
         print(f"[bench] FP64 leg: {K} steps", file=sys.stderr, flush=True)

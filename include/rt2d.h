@@ -145,6 +145,11 @@ int rt_get_moments(rt_ctx *ctx, double *E, double *Fx, double *Fy, double *Pxx, 
  * src/Radiation.cpp:726-733). */
 int rt_iteration_stats(rt_ctx *ctx, int32_t *maxCount, double *sumCount);
 int rt_reset_iteration_stats(rt_ctx *ctx);
+/* Per-cell cycle counts of the last reference-order rt_collide
+ * (Radiation::itterationCount, src/Radiation.cpp:715-724); count gets N
+ * entries, 0 for halo and excised cells.  The fused rt_step keeps only the
+ * aggregate statistics above. */
+int rt_get_iteration_counts(rt_ctx *ctx, int32_t *count);
 
 /* ---- hot path ------------------------------------------------------------ */
 /* nSteps full steps, each stream + moments + collide fused in one kernel.
