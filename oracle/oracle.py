@@ -91,19 +91,19 @@ def lib(fast=False):
     L.or_collide.argtypes = [C.POINTER(Sim), dptr, dptr, dptr, C.c_void_p]
     L.or_num_threads.restype = i32
     L.or_set_threads.argtypes = [i32]
-    if hasattr(L, "or_adjoint_run"):
-        _bind_adjoint(L)
+    _bind_adjoint(L)
     _LIBS[name] = L
     return L
 
 
 def _bind_adjoint(L):
-    d, i32 = C.c_double, C.c_int32
+    i32 = C.c_int32
+    L.or_adj_forward.restype = C.c_int
+    L.or_adj_forward.argtypes = [C.POINTER(Sim), i32, dptr, dptr]
+    L.or_observer_loss.restype = C.c_double
+    L.or_observer_loss.argtypes = [C.POINTER(Sim), dptr, i32, dptr, dptr, dptr, C.c_void_p, C.c_void_p]
     L.or_adjoint_run.restype = C.c_int
-    L.or_adjoint_run.argtypes = [C.POINTER(Sim), i32, i32, dptr, i32, dptr, dptr, dptr, dptr,
-                                 C.POINTER(d), dptr, dptr]
-    L.or_observer_loss.restype = d
-    L.or_observer_loss.argtypes = [C.POINTER(Sim), dptr, i32, dptr, dptr, dptr]
+    L.or_adjoint_run.argtypes = [C.POINTER(Sim), i32, dptr, dptr, dptr, dptr]
 
 
 def make_grid(nx_, ny_, start, end, cfl):
@@ -170,6 +170,37 @@ class Problem:
         s.bh_r2 = bh_r2
         s.eta, s.kappa0, s.kappa1, s.kappaA, s.ux, s.uy = (a.ctypes.data for a in self.fields)
         self.sim = s
+
+    def adj_forward(self, I0, nSteps):
+        """Moment-space forward (u = 0) storing every state: [(nSteps+1), N*nDir]."""
+        ND = self.grid.N * self.nDir
+        states = np.zeros((nSteps + 1) * ND)
+        rc = self.L.or_adj_forward(C.byref(self.sim), nSteps, np.ascontiguousarray(I0, dtype=np.float64), states)
+        assert rc == 0, "adjoint oracle needs zero fluid velocity"
+        return states.reshape(nSteps + 1, ND)
+
+    def observer_loss(self, I, px, py, target, want_grad=False):
+        """Ring loss 1/2 sum (obs - target)^2 over outward directions; returns
+        (L, obs [npts, nDir], dL/dI or None)."""
+        npts = len(px)
+        obs = np.zeros(npts * self.nDir)
+        g = np.zeros(self.grid.N * self.nDir) if want_grad else None
+        L = self.L.or_observer_loss(C.byref(self.sim), np.ascontiguousarray(I, dtype=np.float64), npts,
+                                    np.ascontiguousarray(px, dtype=np.float64),
+                                    np.ascontiguousarray(py, dtype=np.float64),
+                                    np.ascontiguousarray(target, dtype=np.float64).reshape(-1),
+                                    obs.ctypes.data, g.ctypes.data if want_grad else None)
+        return L, obs.reshape(npts, self.nDir), g
+
+    def adjoint(self, states, lamT):
+        """dL/d eta and dL/d kappa0 ([N] each) from stored states and dL/dI_T."""
+        nSteps = states.shape[0] - 1
+        ge = np.zeros(self.grid.N)
+        gk = np.zeros(self.grid.N)
+        rc = self.L.or_adjoint_run(C.byref(self.sim), nSteps, np.ascontiguousarray(states.reshape(-1)),
+                                   np.ascontiguousarray(lamT, dtype=np.float64), ge, gk)
+        assert rc == 0
+        return ge, gk
 
     def run(self, I, nSteps, want_counts=False):
         """Advance I (AoS [N*nDir], float64) by nSteps; returns (I, moments, max, sum, counts)."""
