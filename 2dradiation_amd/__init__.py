@@ -82,6 +82,10 @@ def lib():
     L.rt_collide.argtypes = [vp]
     L.rt_uses_fused_path.argtypes = [vp, C.POINTER(i32)]
     L.rt_time_steps.argtypes = [vp, i32, C.POINTER(C.c_float)]
+    L.rt_get_iteration_counts.argtypes = [vp, _ip]
+    L.rt_forward_checkpointed.argtypes = [vp, i32, i32]
+    L.rt_sample_outgoing.argtypes = [vp, i32, _dp, _dp, _dp]
+    L.rt_adjoint.argtypes = [vp, i32, _dp, _dp, _dp, _dp, _dp]
     L.rt_build_stencil.argtypes = [i32, _dp, _dp, _dp, _dp, vp, vp]
     L.rt_build_metric.argtypes = [i32, d, d, i32, i32, d, d, d, d, vp, vp, vp, vp, vp, vp]
     _lib = L
@@ -235,6 +239,35 @@ class Context:
         f = C.c_int32()
         self._c(lib().rt_uses_fused_path(self.h, C.byref(f)))
         return bool(f.value)
+
+    def iteration_counts(self):
+        """Per-cell cycle counts of the last reference-order collide."""
+        cnt = np.zeros(self.grid.N, dtype=np.int32)
+        self._c(lib().rt_get_iteration_counts(self.h, cnt))
+        return cnt
+
+    def forward_checkpointed(self, n, every):
+        """n steps keeping a checkpoint every `every` steps for adjoint()."""
+        self._c(lib().rt_forward_checkpointed(self.h, n, every))
+
+    def sample_outgoing(self, x, y):
+        """Bilinear samples [npts, nDir] of the current state at (x, y), outward directions only."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros(len(x) * self.nDir)
+        self._c(lib().rt_sample_outgoing(self.h, len(x), x, y, out))
+        return out.reshape(len(x), self.nDir)
+
+    def adjoint(self, x, y, dL_dout):
+        """dL/d eta and dL/d kappa0 ([N] each) for dL/d(sample_outgoing(x, y)) after
+        the last forward_checkpointed run."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        g = np.ascontiguousarray(dL_dout, dtype=np.float64).reshape(-1)
+        ge = np.zeros(self.grid.N)
+        gk = np.zeros(self.grid.N)
+        self._c(lib().rt_adjoint(self.h, len(x), x, y, g, ge, gk))
+        return ge, gk
 
     def time_steps(self, n):
         ms = C.c_float()

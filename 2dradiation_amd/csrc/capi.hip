@@ -3,6 +3,7 @@
  * include/rt2d.h).  Host-side orchestration only; kernels live in
  * kernels_ref.hip and kernels_fused.hip.
  */
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -52,9 +53,6 @@ size_t tsize(const rt_ctx *c) { return c->fp32 ? sizeof(float) : sizeof(double);
         if (!(p)) return rt_fail((c), RT_ERR_ARG, "null pointer argument: " #p);                    \
     } while (0)
 
-/* Builds the per-direction constants of the fused kernel from the stencil and,
- * for the curved path, checks that the cubic tables repeat with period 10 fine
- * angles (true for the uniform stencils Stencil(n,0) builds). */
 /* Rounds five weights to multiples of 2^-23 (exact in FP32 for |w| < 2) and
  * puts the remainder of target - sum on the largest one, so the FP32 weights
  * sum to target exactly. */
@@ -72,6 +70,9 @@ void exact_sum_f32(const double w[5], double target, float out[5])
     for (int m = 0; m < 5; m++) out[m] = (float)((double)k[m] * q);
 }
 
+/* Builds the per-direction constants of the fused kernel from the stencil and,
+ * for the curved path, checks that the cubic tables repeat with period 10 fine
+ * angles (true for the uniform stencils Stencil(n,0) builds). */
 int build_dir_tables(rt_ctx *c, const double *w, const double *phi, const double *cx, const double *cy,
                      const int32_t *Nc, const double *Wc)
 {
@@ -282,7 +283,9 @@ void rt_destroy(rt_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_w); dfree(c->d_phi); dfree(c->d_cx); dfree(c->d_cy); dfree(c->d_Nc); dfree(c->d_Wc);
     dfree(c->d_basis); dfree(c->d_dirt); dfree(c->d_alpha); dfree(c->d_t11); dfree(c->d_t12);
-    dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_excl); dfree(c->d_red); dfree(c->d_mask); dfree(c->d_f64);
+    dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_excl); dfree(c->d_red);
+    dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
+    dfree(c->d_lam[0]); dfree(c->d_lam[1]); dfree(c->d_grad[0]); dfree(c->d_grad[1]); dfree(c->d_pts); dfree(c->d_mask); dfree(c->d_f64);
     dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_mom); dfree(c->d_count);
     dfree(c->d_stats);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -357,6 +360,7 @@ int rt_set_streaming_stencil(rt_ctx *c, const double *w, const double *phi, cons
 int rt_set_geometry(rt_ctx *c, const double *alpha, const double *t11, const double *t12, const double *t21, const double *t22)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    c->ck_valid = false;
     NEED(c, alpha); NEED(c, t11); NEED(c, t12); NEED(c, t21); NEED(c, t22);
     const size_t N = c->N;
     const rt_desc &d = c->desc;
@@ -455,6 +459,7 @@ int rt_set_fields(rt_ctx *c, const double *eta, const double *kappa0, const doub
                   const double *uy)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    c->ck_valid = false;
     const double *src[6] = {eta, kappa0, kappa1, kappaA, ux, uy};
     const size_t N = c->N;
     bool any_u = false;
@@ -479,6 +484,7 @@ int rt_set_fields(rt_ctx *c, const double *eta, const double *kappa0, const doub
 int rt_upload_I(rt_ctx *c, const double *I)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    c->ck_valid = false;
     NEED(c, I);
     const size_t n = (size_t)c->N * c->D;
     if (!all_finite(I, n)) return rt_fail(c, RT_ERR_NONFINITE, "intensities contain NaN/Inf");
@@ -573,6 +579,18 @@ static int ensure_unscaled(rt_ctx *c)
     return RT_OK;
 }
 
+/* State convention of the fused kernel: curved state as alpha^-3 I, and the
+ * halo and excised cells equal in both buffers. */
+static int prepare_fused(rt_ctx *c)
+{
+    if (c->desc.streaming == RT_CURVED_FIXED && !c->scaled) {
+        TRY(launch_scale_state(c, c->d_I[c->cur], false));
+        c->scaled = true;
+    }
+    TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+    return RT_OK;
+}
+
 static int check_ready(rt_ctx *c)
 {
     if (!c->has_stencil) return rt_fail(c, RT_ERR_STATE, "rt_set_stencil has not been called");
@@ -594,16 +612,12 @@ int rt_uses_fused_path(rt_ctx *c, int32_t *fused)
 int rt_step(rt_ctx *c, int32_t nSteps, int32_t want_moments)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    c->ck_valid = false;
     if (nSteps < 0) return rt_fail(c, RT_ERR_ARG, "nSteps must be >= 0");
     TRY(check_ready(c));
     if (nSteps == 0) return RT_OK;
     if (fused_supported(c)) {
-        if (c->desc.streaming == RT_CURVED_FIXED && !c->scaled) {
-            TRY(launch_scale_state(c, c->d_I[c->cur], false));
-            c->scaled = true;
-        }
-        /* halo and excised cells are the same in both buffers from here on */
-        TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+        TRY(prepare_fused(c));
         if (want_moments) TRY(launch_fill(c, c->d_mom, 6 * (size_t)c->N, 0.0));
         for (int n = 0; n < nSteps; n++) {
             TRY(launch_fused_step(c, c->d_I[c->cur], c->d_I[1 - c->cur], want_moments && n == nSteps - 1));
@@ -626,6 +640,7 @@ int rt_step(rt_ctx *c, int32_t nSteps, int32_t want_moments)
 int rt_stream(rt_ctx *c)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    c->ck_valid = false;
     TRY(check_ready(c));
     TRY(ensure_unscaled(c));
     TRY(launch_ref_stream(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
@@ -642,6 +657,7 @@ int rt_compute_moments(rt_ctx *c)
 int rt_collide(rt_ctx *c)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    c->ck_valid = false;
     TRY(check_ready(c));
     TRY(ensure_unscaled(c));
     TRY(launch_ref_collide(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
@@ -657,6 +673,152 @@ int rt_time_steps(rt_ctx *c, int32_t nSteps, float *ms)
     RT_HIP(c, hipEventRecord(c->ev1, c->stream));
     RT_HIP(c, hipEventSynchronize(c->ev1));
     RT_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+    return RT_OK;
+}
+
+/* ---- adjoint ------------------------------------------------------------ */
+
+static int check_adjoint_ready(rt_ctx *c)
+{
+    TRY(check_ready(c));
+    if (c->any_u) return rt_fail(c, RT_ERR_UNSUPPORTED, "the adjoint needs zero fluid velocity");
+    if (!fused_supported(c)) return rt_fail(c, RT_ERR_UNSUPPORTED, "the adjoint needs the fused step (stencil or time step not covered)");
+    return RT_OK;
+}
+
+int rt_forward_checkpointed(rt_ctx *c, int32_t nSteps, int32_t every)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    c->ck_valid = false;
+    if (nSteps < 0 || every < 1) return rt_fail(c, RT_ERR_ARG, "nSteps must be >= 0 and every >= 1");
+    TRY(check_adjoint_ready(c));
+    const size_t ND = (size_t)c->N * c->D, bytes = ND * tsize(c);
+    const int count = (nSteps + every - 1) / every;
+    if (count > c->ck_cap) {
+        dfree(c->d_ckpt);
+        TRY(dalloc(c, &c->d_ckpt, (size_t)count * bytes));
+        c->ck_cap = count;
+    }
+    TRY(prepare_fused(c));
+    for (int s = 0; s < count; s++) {
+        RT_HIP(c, hipMemcpyAsync((char *)c->d_ckpt + (size_t)s * bytes, c->d_I[c->cur], bytes, hipMemcpyDeviceToDevice,
+                                 c->stream));
+        const int steps = std::min(every, nSteps - s * every);
+        for (int n = 0; n < steps; n++) {
+            TRY(launch_fused_step(c, c->d_I[c->cur], c->d_I[1 - c->cur], false));
+            c->cur = 1 - c->cur;
+        }
+    }
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    c->ck_steps = nSteps;
+    c->ck_every = every;
+    c->ck_count = count;
+    c->ck_valid = true;
+    return RT_OK;
+}
+
+/* d_pts layout: x [npts], y [npts], then npts*nDir values (samples or their
+ * gradients) */
+static int upload_points(rt_ctx *c, int32_t npts, const double *x, const double *y, const double *g)
+{
+    const int need = 2 * npts + npts * c->D;
+    if (need > c->pts_cap) {
+        dfree(c->d_pts);
+        TRY(dalloc(c, &c->d_pts, (size_t)need * sizeof(double)));
+        c->pts_cap = need;
+    }
+    RT_HIP(c, hipMemcpyAsync(c->d_pts, x, npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(c->d_pts + npts, y, npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (g)
+        RT_HIP(c, hipMemcpyAsync(c->d_pts + 2 * npts, g, (size_t)npts * c->D * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+    return RT_OK;
+}
+
+static int check_points(rt_ctx *c, int32_t npts, const double *x, const double *y)
+{
+    if (npts < 1) return rt_fail(c, RT_ERR_ARG, "npts must be >= 1");
+    const rt_desc &d = c->desc;
+    for (int p = 0; p < npts; p++) {
+        const double i = (x[p] - d.startx) / d.dx, j = (y[p] - d.starty) / d.dy;
+        if (!(i >= 0 && j >= 0 && i < d.nx - 1 && j < d.ny - 1))
+            return rt_fail(c, RT_ERR_ARG, "sample point outside the grid");
+    }
+    return RT_OK;
+}
+
+int rt_sample_outgoing(rt_ctx *c, int32_t npts, const double *x, const double *y, double *out)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    NEED(c, x); NEED(c, y); NEED(c, out);
+    TRY(check_points(c, npts, x, y));
+    TRY(upload_points(c, npts, x, y, nullptr));
+    double *dout = c->d_pts + 2 * npts;
+    TRY(launch_sample(c, c->d_I[c->cur], c->scaled ? 1 : 0, npts, c->d_pts, c->d_pts + npts, dout));
+    RT_HIP(c, hipMemcpyAsync(out, dout, (size_t)npts * c->D * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const double *dL_dout, double *grad_eta,
+               double *grad_kappa0)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    NEED(c, x); NEED(c, y); NEED(c, dL_dout); NEED(c, grad_eta); NEED(c, grad_kappa0);
+    if (!c->ck_valid) return rt_fail(c, RT_ERR_STATE, "rt_adjoint needs a preceding rt_forward_checkpointed run");
+    TRY(check_adjoint_ready(c));
+    TRY(check_points(c, npts, x, y));
+    if (!all_finite(dL_dout, (size_t)npts * c->D)) return rt_fail(c, RT_ERR_NONFINITE, "dL_dout contains NaN/Inf");
+    const size_t N = c->N, ND = N * c->D, bytes = ND * tsize(c);
+    const int every = c->ck_every;
+    if (every > c->seg_cap) {
+        dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
+        TRY(dalloc(c, &c->d_seg, (size_t)every * bytes));
+        TRY(dalloc(c, &c->d_rec_m, (size_t)every * 6 * N * sizeof(double)));
+        TRY(dalloc(c, &c->d_rec_k, (size_t)every * N));
+        TRY(dalloc(c, &c->d_rec_mask, (size_t)every * RT_FUSED_WAVES * N * sizeof(unsigned long long)));
+        c->seg_cap = every;
+    }
+    TRY(dalloc(c, &c->d_lam[0], bytes));
+    TRY(dalloc(c, &c->d_lam[1], bytes));
+    TRY(dalloc(c, &c->d_grad[0], N * sizeof(double)));
+    TRY(dalloc(c, &c->d_grad[1], N * sizeof(double)));
+    TRY(upload_points(c, npts, x, y, dL_dout));
+    RT_HIP(c, hipMemsetAsync(c->d_grad[0], 0, N * sizeof(double), c->stream));
+    RT_HIP(c, hipMemsetAsync(c->d_grad[1], 0, N * sizeof(double), c->stream));
+    int lc = 0;
+    RT_HIP(c, hipMemsetAsync(c->d_lam[lc], 0, bytes, c->stream));
+    /* dL/d(final state) from the sampled outputs */
+    TRY(launch_sample_adjoint(c, c->d_lam[lc], c->scaled ? 1 : 0, npts, c->d_pts, c->d_pts + npts,
+                              c->d_pts + 2 * npts));
+    auto rec_at = [&](int i) {
+        RecordPtrs r;
+        r.m = c->d_rec_m + (size_t)i * 6 * N;
+        r.k = c->d_rec_k + (size_t)i * N;
+        r.mask = c->d_rec_mask + (size_t)i * RT_FUSED_WAVES * N;
+        return r;
+    };
+    for (int s = c->ck_count - 1; s >= 0; s--) {
+        const int steps = std::min(every, c->ck_steps - s * every);
+        /* recompute the segment, keeping every state after a step */
+        const void *prev = (const char *)c->d_ckpt + (size_t)s * bytes;
+        for (int i = 0; i < steps; i++) {
+            void *out = (char *)c->d_seg + (size_t)i * bytes;
+            TRY(launch_copy_fixed_cells(c, prev, out));
+            TRY(launch_fused_step_rec(c, prev, out, rec_at(i)));
+            prev = out;
+        }
+        /* adjoint steps, last to first */
+        for (int i = steps - 1; i >= 0; i--) {
+            RT_HIP(c, hipMemsetAsync(c->d_lam[1 - lc], 0, bytes, c->stream));
+            TRY(launch_adjoint_step(c, c->d_lam[lc], c->d_lam[1 - lc], (const char *)c->d_seg + (size_t)i * bytes,
+                                    rec_at(i)));
+            lc = 1 - lc;
+        }
+    }
+    RT_HIP(c, hipMemcpyAsync(grad_eta, c->d_grad[0], N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipMemcpyAsync(grad_kappa0, c->d_grad[1], N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
     return RT_OK;
 }
 

@@ -1,0 +1,148 @@
+/*
+ * Device helpers shared by the fused forward step and its adjoint: the
+ * per-direction geometry of the curved gather (Fourier series at the stencil
+ * angle, departure cell, cubic angular window) and the direct-to-LDS copy.
+ */
+#ifndef RT_FUSED_COMMON_H
+#define RT_FUSED_COMMON_H
+#include "rt_internal.h"
+
+namespace rtf {
+
+template <typename T>
+__device__ inline T fast_div(T a, T b)
+{
+    return a / b;
+}
+template <>
+__device__ inline float fast_div<float>(float a, float b)
+{
+    return a * __builtin_amdgcn_rcpf(b);
+}
+
+template <typename T>
+__device__ inline T tmax0(T v)
+{
+    return v > T(0) ? v : T(0);
+}
+
+/* Per-direction tables read with wave-uniform indices: a pointer into the
+ * constant address space lets the compiler use scalar loads. */
+typedef const __attribute__((address_space(4))) DirTables CDirTables;
+
+/* Copies n dwords from global src to LDS dst with one dword per lane per
+ * instruction; lanes past n do nothing.  dst must be wave-uniform. */
+__device__ inline void lds_dma_dwords(const void *src, void *dst, int n, int lane)
+{
+    const uint32_t *g = (const uint32_t *)src;
+    uint32_t *l = (uint32_t *)dst;
+    for (int b = 0; b < n; b += 64) {
+        if (b + lane < n) __builtin_amdgcn_global_load_lds(g + b + lane, l + b, 4, 0, 0);
+    }
+}
+
+/* Fine angular index nGrid*phi/(2 pi) of Tensor2::Phi (src/DataTypes.hh:166-170)
+ * with the Hastings atan2 (src/Utility.hh:220-254), branch-free.  The octant
+ * offsets are exact multiples of nGrid/4, so only the polynomial term rounds. */
+template <typename T>
+__device__ inline T angle_index_bf(T vx, T vy, T u, int quarter)
+{
+    const bool swap = !(fabsf((float)vx) > fabsf((float)vy));
+    const T num = swap ? vx : vy, den = swap ? vy : vx;
+    const T p = rtm::hastings_atan(fast_div(num, den));
+    /* octant offset in quarters: |vx| > |vy|: 0 (or 4 when p < 0) for vx > 0,
+     * 2 for vx < 0; otherwise 1 for vy > 0, 3 for vy < 0 (and the
+     * polynomial enters with a minus sign) */
+    const int px = (int)(vx > T(0)), pneg = (int)(p < T(0)), py = (int)(vy > T(0));
+    const int kx = 2 - 2 * px + 4 * (px & pneg);
+    const int ky = 3 - 2 * py;
+    const int k = swap ? ky : kx;
+    const T up = u * p;
+    const T dd = T(k * quarter) + (swap ? -up : up);
+    return (vx == T(0) && vy == T(0)) ? T(0) : dd;
+}
+
+/* Geometry of one (cell, direction) of the curved gather: the ray factor as
+ * s - 1, the departure cell offset (fi, fj in {-1, 0}) and bilinear weights,
+ * and the 5 combined angular taps starting at direction s0 (0 <= s0 < D).
+ * Every index is clamped into range for any input, NaN included.
+ * Reference: StreamCurvedFixed (src/Radiation.cpp:454-512) and IntensityAt
+ * (:322-343).  wrow: the FP32 weight rows in LDS (FP32 only). */
+template <typename T>
+struct DirGeom {
+    T s, fx, fy;
+    T Cw[5];
+    int s0;
+    bool lo_i, lo_j;
+};
+
+template <typename T, int D>
+__device__ inline void curved_geometry(const T (&cS)[5], const T (&cX)[5], const T (&cY)[5], const T (&cU)[5],
+                                       const T (&cV)[5], CDirTables *dtab, CDirTables *dtab0, const float *wrow,
+                                       int d, T angu, int quarter, int nGrid, T ddmax, DirGeom<T> &G)
+{
+    T B[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) B[q] = (sizeof(T) == 4) ? (T)dtab->basisf[d][q] : (T)dtab->basis[d][q];
+    T s = 0, di = 0, dj = 0, vx = 0, vy = 0;
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        s += cS[q] * B[q];
+        di += cX[q] * B[q];
+        dj += cY[q] * B[q];
+        vx += cU[q] * B[q];
+        vy += cV[q] * B[q];
+    }
+    G.s = s;
+    const T fi = fmin(fmax(floor(di), T(-1)), T(0));
+    const T fj = fmin(fmax(floor(dj), T(-1)), T(0));
+    G.fx = di - fi;
+    G.fy = dj - fj;
+    G.lo_i = fi < T(0);
+    G.lo_j = fj < T(0);
+    T dd = angle_index_bf<T>(vx, vy, angu, quarter);
+    dd = fmin(fmax(dd, T(1e-8)), ddmax);
+    int g0 = (int)dd;
+    const T t = dd - T(g0);
+    g0 = (unsigned)g0 < (unsigned)nGrid ? g0 : 0; /* dd == nGrid after rounding is angle 0 */
+    const int q10 = (int)(__umul24((unsigned)g0, 6554u) >> 16); /* g0 / 10 for g0 < 16384 */
+    const int r = g0 - 10 * q10;
+    int woff;
+    if constexpr (sizeof(T) == 4) {
+        const float4 *wr = (const float4 *)((const char *)wrow + __umul24((unsigned)r, 48u));
+        const float4 w0 = wr[0], w1 = wr[1], w2 = wr[2];
+        G.Cw[0] = w0.x + t * w1.y;
+        G.Cw[1] = w0.y + t * w1.z;
+        G.Cw[2] = w0.z + t * w1.w;
+        G.Cw[3] = w0.w + t * w2.x;
+        G.Cw[4] = w1.x + t * w2.y;
+        woff = __float_as_int(w2.z);
+    } else {
+        /* FP64 rows come through the vector L1 (faster than LDS for 8-byte
+         * elements) */
+#pragma unroll
+        for (int m = 0; m < 5; m++) G.Cw[m] = dtab0->wA[r][m] + t * dtab0->wD[r][m];
+        woff = dtab0->woff[r];
+    }
+    int s0 = q10 + woff;
+    G.s0 = s0 < 0 ? s0 + D : (s0 >= D ? s0 - D : s0);
+}
+
+/* The 25 FP64 Fourier coefficients of a cell from LDS (C[series*5+k][lane]),
+ * as S - 1 and X, Y relative to the cell centre in cell units. */
+template <typename T>
+__device__ inline void cell_series(const double (*C)[64], int lane, double xc, double yc, double idx, double idy,
+                                   T (&cS)[5], T (&cX)[5], T (&cY)[5], T (&cU)[5], T (&cV)[5])
+{
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        cS[k] = (T)(k == 0 ? C[0][lane] - 1.0 : C[k][lane]);
+        cX[k] = (T)((k == 0 ? C[5][lane] - xc : C[5 + k][lane]) * idx);
+        cY[k] = (T)((k == 0 ? C[10][lane] - yc : C[10 + k][lane]) * idy);
+        cU[k] = (T)C[15 + k][lane];
+        cV[k] = (T)C[20 + k][lane];
+    }
+}
+
+} // namespace rtf
+#endif

@@ -35,9 +35,10 @@
  * ray, so the angular weight rows sum exactly to 1 and 0 in FP32 and the
  * ray factor s enters as s - 1.
  */
-#include "rt_internal.h"
+#include "fused_common.h"
 
 namespace {
+using namespace rtf;
 
 template <typename T>
 struct KP {
@@ -55,28 +56,12 @@ struct KP {
     int strips, chunks, rows_per_block, nblocks;
     double startx, starty, dx, dy, dtime, r2;
     int want_moments;
+    /* adjoint recording (k_fused<..., REC = true>): moments of the streamed
+     * intensities [6][N], lambda cycle counts [N], corner clamp bits [4][N] */
+    double *rec_m;
+    uint8_t *rec_k;
+    unsigned long long *rec_mask;
 };
-
-template <typename T>
-__device__ inline T fast_div(T a, T b)
-{
-    return a / b;
-}
-template <>
-__device__ inline float fast_div<float>(float a, float b)
-{
-    return a * __builtin_amdgcn_rcpf(b);
-}
-
-template <typename T>
-__device__ inline T tmax0(T v)
-{
-    return v > T(0) ? v : T(0);
-}
-
-/* Per-direction tables read with wave-uniform indices: a pointer into the
- * constant address space lets the compiler use scalar loads. */
-typedef const __attribute__((address_space(4))) DirTables CDirTables;
 
 /* LDS of one workgroup.  Intensity rows are SoA [direction][66 columns]
  * with 4 wrap-around direction rows (D..D+3 repeat 0..3), so the 5 angular
@@ -92,38 +77,6 @@ struct Smem {
     double X[RT_FUSED_WAVES][6][64];
     float wrow[10][12] __attribute__((aligned(16)));
 };
-
-/* Copies n dwords from global src to LDS dst with one dword per lane per
- * instruction; lanes past n do nothing.  dst must be wave-uniform. */
-__device__ inline void lds_dma_dwords(const void *src, void *dst, int n, int lane)
-{
-    const uint32_t *g = (const uint32_t *)src;
-    uint32_t *l = (uint32_t *)dst;
-    for (int b = 0; b < n; b += 64) {
-        if (b + lane < n) __builtin_amdgcn_global_load_lds(g + b + lane, l + b, 4, 0, 0);
-    }
-}
-
-/* Fine angular index nGrid*phi/(2 pi) of Tensor2::Phi (src/DataTypes.hh:166-170)
- * with the Hastings atan2 (src/Utility.hh:220-254), branch-free.  The octant
- * offsets are exact multiples of nGrid/4, so only the polynomial term rounds. */
-template <typename T>
-__device__ inline T angle_index_bf(T vx, T vy, T u, int quarter)
-{
-    const bool swap = !(fabsf((float)vx) > fabsf((float)vy));
-    const T num = swap ? vx : vy, den = swap ? vy : vx;
-    const T p = rtm::hastings_atan(fast_div(num, den));
-    /* octant offset in quarters: |vx| > |vy|: 0 (or 4 when p < 0) for vx > 0,
-     * 2 for vx < 0; otherwise 1 for vy > 0, 3 for vy < 0 (and the
-     * polynomial enters with a minus sign) */
-    const int px = (int)(vx > T(0)), pneg = (int)(p < T(0)), py = (int)(vy > T(0));
-    const int kx = 2 - 2 * px + 4 * (px & pneg);
-    const int ky = 3 - 2 * py;
-    const int k = swap ? ky : kx;
-    const T up = u * p;
-    const T dd = T(k * quarter) + (swap ? -up : up);
-    return (vx == T(0) && vy == T(0)) ? T(0) : dd;
-}
 
 /* Reference lambda cycle for rows that contain a moving cell
  * (src/Radiation.cpp:661-722), with a workgroup moment exchange per cycle.
@@ -203,7 +156,7 @@ constexpr int fused_min_blocks()
     return sizeof(Smem<T, D, CURVED>) <= 80 * 1024 ? 2 : 1;
 }
 
-template <typename T, int D, bool CURVED>
+template <typename T, int D, bool CURVED, bool REC>
 __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fused(KP<T> p)
 {
     constexpr int DW = D / RT_FUSED_WAVES;
@@ -302,22 +255,13 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
         /* ---- stream ---- */
         T sv[DW];
         double acc[6] = {0, 0, 0, 0, 0, 0};
+        unsigned long long mbits = 0; /* REC: bit 4k+c set when corner c of direction k is not clamped */
         if constexpr (CURVED) {
             /* geodesic Fourier coefficients of this cell: S as S - 1, X and Y
              * relative to the cell centre in cell units */
             T cS[5], cX[5], cY[5], cU[5], cV[5];
-            {
-                const double xc = p.startx + gi * p.dx, yc = p.starty + j * p.dy;
-                const double idx = 1.0 / p.dx, idy = 1.0 / p.dy;
-#pragma unroll
-                for (int k = 0; k < 5; k++) {
-                    cS[k] = (T)(k == 0 ? sm.C[0][lane] - 1.0 : sm.C[k][lane]);
-                    cX[k] = (T)((k == 0 ? sm.C[5][lane] - xc : sm.C[5 + k][lane]) * idx);
-                    cY[k] = (T)((k == 0 ? sm.C[10][lane] - yc : sm.C[10 + k][lane]) * idy);
-                    cU[k] = (T)sm.C[15 + k][lane];
-                    cV[k] = (T)sm.C[20 + k][lane];
-                }
-            }
+            cell_series<T>(sm.C, lane, p.startx + gi * p.dx, p.starty + j * p.dy, 1.0 / p.dx, 1.0 / p.dy, cS, cX, cY,
+                           cU, cV);
             /* directions of this wavefront whose departure point is excised */
             const uint32_t exw = (uint32_t)(__double_as_longlong(sm.C[25][lane]) >> (wv * DW));
             const T angu = (sizeof(T) == 4) ? (T)dtab->ang_uf : (T)dtab->ang_u;
@@ -339,54 +283,18 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
             };
             auto prep = [&](int k, Prep &P) {
                 const int d = wv * DW + k;
-                T B[5];
-#pragma unroll
-                for (int q = 0; q < 5; q++) B[q] = (sizeof(T) == 4) ? (T)dtab->basisf[d][q] : (T)dtab->basis[d][q];
-                T s = 0, di = 0, dj = 0, vx = 0, vy = 0;
-#pragma unroll
-                for (int q = 0; q < 5; q++) {
-                    s += cS[q] * B[q];
-                    di += cX[q] * B[q];
-                    dj += cY[q] * B[q];
-                    vx += cU[q] * B[q];
-                    vy += cV[q] * B[q];
-                }
-                P.s = s;
+                DirGeom<T> G;
+                curved_geometry<T, D>(cS, cX, cY, cU, cV, dtab, dtab0, &sm.wrow[0][0], d, angu, quarter, nGrid, ddmax,
+                                      G);
+                P.s = G.s;
                 P.go = active && !((exw >> k) & 1u);
-                const T fi = fmin(fmax(floor(di), T(-1)), T(0));
-                const T fj = fmin(fmax(floor(dj), T(-1)), T(0));
-                P.fx = di - fi;
-                P.fy = dj - fj;
-                T dd = angle_index_bf<T>(vx, vy, angu, quarter);
-                dd = fmin(fmax(dd, T(1e-8)), ddmax);
-                int g0 = (int)dd;
-                const T t = dd - T(g0);
-                g0 = (unsigned)g0 < (unsigned)nGrid ? g0 : 0; /* dd == nGrid after rounding is angle 0 */
-                const int q10 = (int)(__umul24((unsigned)g0, 6554u) >> 16); /* g0 / 10 for g0 < 16384 */
-                const int r = g0 - 10 * q10;
-                int woff;
-                if constexpr (sizeof(T) == 4) {
-                    const float4 *wr = (const float4 *)((const char *)&sm.wrow[0][0] + __umul24((unsigned)r, 48u));
-                    const float4 w0 = wr[0], w1 = wr[1], w2 = wr[2];
-                    P.Cw[0] = w0.x + t * w1.y;
-                    P.Cw[1] = w0.y + t * w1.z;
-                    P.Cw[2] = w0.z + t * w1.w;
-                    P.Cw[3] = w0.w + t * w2.x;
-                    P.Cw[4] = w1.x + t * w2.y;
-                    woff = __float_as_int(w2.z);
-                } else {
-                    /* FP64 rows come through the vector L1 (faster than LDS
-                     * for 8-byte elements) */
+                P.fx = G.fx;
+                P.fy = G.fy;
 #pragma unroll
-                    for (int m = 0; m < 5; m++) P.Cw[m] = dtab0->wA[r][m] + t * dtab0->wD[r][m];
-                    woff = dtab0->woff[r];
-                }
-                int s0 = q10 + woff;
-                s0 = s0 < 0 ? s0 + D : (s0 >= D ? s0 - D : s0);
-                const bool lo_j = fj < T(0);
-                const T *ra = lo_j ? colm : col0p;
-                const T *rb = lo_j ? col0p : colp;
-                const int off = (int)__umul24((unsigned)s0, (unsigned)RW) + (fi < T(0) ? 0 : 1);
+                for (int m = 0; m < 5; m++) P.Cw[m] = G.Cw[m];
+                const T *ra = G.lo_j ? colm : col0p;
+                const T *rb = G.lo_j ? col0p : colp;
+                const int off = (int)__umul24((unsigned)G.s0, (unsigned)RW) + (G.lo_i ? 0 : 1);
                 const T *pa = ra + off;
                 const T *pb = rb + off;
 #pragma unroll
@@ -410,6 +318,11 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
                 /* state is alpha^-3 I: the (alpha/alpha_k)^3 factors cancel.  s is
                  * carried as s - 1, so s^3 x = x + (s^3 - 1) x rounds once, at the end */
                 const T bil = rtm::bilinear<T>(P.fx, P.fy, tmax0(v00), tmax0(v01), tmax0(v10), tmax0(v11));
+                if constexpr (REC) {
+                    const unsigned bits = (unsigned)(v00 > T(0)) | ((unsigned)(v01 > T(0)) << 1) |
+                                          ((unsigned)(v10 > T(0)) << 2) | ((unsigned)(v11 > T(0)) << 3);
+                    mbits |= (unsigned long long)bits << (4 * k);
+                }
                 const T s3m1 = P.s * (T(3) + P.s * (T(3) + P.s));
                 const T val = P.go ? bil + bil * s3m1 : T(0);
                 sv[k] = val;
@@ -550,6 +463,16 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
         }
 
         /* ---- write back ---- */
+        if constexpr (REC) {
+            if (active) {
+                if (wv == 0) {
+#pragma unroll
+                    for (int X = 0; X < 6; X++) (p.rec_m + (size_t)X * N)[ij] = mI[X];
+                    p.rec_k[ij] = (uint8_t)cycles;
+                }
+                if (CURVED) (p.rec_mask + (size_t)wv * N)[ij] = mbits;
+            }
+        }
         if (active) {
 #pragma unroll
             for (int k = 0; k < DW; k++) (p.Iout + (size_t)(wv * DW + k) * N)[ij] = sv[k];
@@ -577,8 +500,8 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
     }
 }
 
-template <typename T, int D, bool CURVED>
-int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments)
+template <typename T, int D, bool CURVED, bool REC>
+int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments, const RecordPtrs *rec)
 {
     const rt_desc &d = c->desc;
     KP<T> p;
@@ -611,21 +534,24 @@ int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments)
     p.dtime = d.dt;
     p.r2 = d.excision_r2;
     p.want_moments = want_moments ? 1 : 0;
-    k_fused<T, D, CURVED><<<p.nblocks, 256, 0, c->stream>>>(p);
+    p.rec_m = rec ? rec->m : nullptr;
+    p.rec_k = rec ? rec->k : nullptr;
+    p.rec_mask = rec ? rec->mask : nullptr;
+    k_fused<T, D, CURVED, REC><<<p.nblocks, 256, 0, c->stream>>>(p);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }
 
-template <typename T, bool CURVED>
-int dispatch_d(rt_ctx *c, const void *in, void *out, bool wm)
+template <typename T, bool CURVED, bool REC>
+int dispatch_d(rt_ctx *c, const void *in, void *out, bool wm, const RecordPtrs *rec)
 {
     switch (c->D) {
-    case 16: return launch_one<T, 16, CURVED>(c, in, out, wm);
-    case 20: return launch_one<T, 20, CURVED>(c, in, out, wm);
-    case 24: return launch_one<T, 24, CURVED>(c, in, out, wm);
-    case 32: return launch_one<T, 32, CURVED>(c, in, out, wm);
-    case 48: return launch_one<T, 48, CURVED>(c, in, out, wm);
-    case 64: return launch_one<T, 64, CURVED>(c, in, out, wm);
+    case 16: return launch_one<T, 16, CURVED, REC>(c, in, out, wm, rec);
+    case 20: return launch_one<T, 20, CURVED, REC>(c, in, out, wm, rec);
+    case 24: return launch_one<T, 24, CURVED, REC>(c, in, out, wm, rec);
+    case 32: return launch_one<T, 32, CURVED, REC>(c, in, out, wm, rec);
+    case 48: return launch_one<T, 48, CURVED, REC>(c, in, out, wm, rec);
+    case 64: return launch_one<T, 64, CURVED, REC>(c, in, out, wm, rec);
     default: return rt_fail(c, RT_ERR_UNSUPPORTED, "fused kernel: unsupported nDir");
     }
 }
@@ -650,6 +576,19 @@ bool fused_supported(const rt_ctx *c)
 int launch_fused_step(rt_ctx *c, const void *in, void *out, bool want_moments)
 {
     const bool curved = c->desc.streaming == RT_CURVED_FIXED;
-    if (c->fp32) return curved ? dispatch_d<float, true>(c, in, out, want_moments) : dispatch_d<float, false>(c, in, out, want_moments);
-    return curved ? dispatch_d<double, true>(c, in, out, want_moments) : dispatch_d<double, false>(c, in, out, want_moments);
+    if (c->fp32)
+        return curved ? dispatch_d<float, true, false>(c, in, out, want_moments, nullptr)
+                      : dispatch_d<float, false, false>(c, in, out, want_moments, nullptr);
+    return curved ? dispatch_d<double, true, false>(c, in, out, want_moments, nullptr)
+                  : dispatch_d<double, false, false>(c, in, out, want_moments, nullptr);
+}
+
+int launch_fused_step_rec(rt_ctx *c, const void *in, void *out, const RecordPtrs &rec)
+{
+    const bool curved = c->desc.streaming == RT_CURVED_FIXED;
+    if (c->fp32)
+        return curved ? dispatch_d<float, true, true>(c, in, out, false, &rec)
+                      : dispatch_d<float, false, true>(c, in, out, false, &rec);
+    return curved ? dispatch_d<double, true, true>(c, in, out, false, &rec)
+                  : dispatch_d<double, false, true>(c, in, out, false, &rec);
 }

@@ -86,6 +86,21 @@ struct rt_ctx {
     int32_t *d_count = nullptr;   /* [N] per-method collide cycle counts */
     unsigned long long *d_stats = nullptr; /* [2] sum, max */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    /* adjoint: checkpoints of the last rt_forward_checkpointed run (states
+     * before steps 0, every, 2 every, ...), the recomputed segment, per-step
+     * records, adjoint state buffers and the parameter gradients */
+    bool ck_valid = false;
+    int ck_steps = 0, ck_every = 0, ck_count = 0, ck_cap = 0, seg_cap = 0;
+    void *d_ckpt = nullptr;
+    void *d_seg = nullptr;
+    double *d_rec_m = nullptr;
+    uint8_t *d_rec_k = nullptr;
+    unsigned long long *d_rec_mask = nullptr;
+    void *d_lam[2] = {nullptr, nullptr};
+    double *d_grad[2] = {nullptr, nullptr};
+    double *d_pts = nullptr; /* sample points and their gradients */
+    int pts_cap = 0;
 };
 
 int rt_fail(rt_ctx *c, int code, const std::string &msg);
@@ -100,6 +115,13 @@ int rt_hip_check(rt_ctx *c, hipError_t e, const char *what);
 int launch_transpose_in(rt_ctx *c, const double *aos, void *soa);
 int launch_transpose_out(rt_ctx *c, const void *soa, double *aos);
 int launch_copy_fixed_cells(rt_ctx *c, const void *src, void *dst);
+/* Per-step records the adjoint needs from a forward step. */
+struct RecordPtrs {
+    double *m;                /* [6][N] moments of the streamed intensities */
+    uint8_t *k;               /* [N] lambda cycles */
+    unsigned long long *mask; /* [4][N] corner clamp bits (curved) */
+};
+int launch_fused_step_rec(rt_ctx *c, const void *in, void *out, const RecordPtrs &rec);
 /* state <- I alpha^-3 (inverse = false) or I <- state alpha^3 (inverse = true)
  * for every cell with a finite positive lapse */
 int launch_scale_state(rt_ctx *c, void *buf, bool inverse);
@@ -112,4 +134,8 @@ bool fused_supported(const rt_ctx *c);
 int launch_geodesic_tables(rt_ctx *c, const double *d_metric, double tol);
 int launch_departure_bound(rt_ctx *c, double *max_disp);
 int launch_fill(rt_ctx *c, double *p, size_t n, double v);
+int launch_adjoint_step(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, const RecordPtrs &rec);
+int launch_sample(rt_ctx *c, const void *I, int scaled, int npts, const double *px, const double *py, double *out);
+int launch_sample_adjoint(rt_ctx *c, void *lam, int scaled, int npts, const double *px, const double *py,
+                          const double *g);
 #endif

@@ -168,6 +168,27 @@ int rt_collide(rt_ctx *ctx);
  * kernels (for stencils or time steps the fused kernel does not cover). */
 int rt_uses_fused_path(rt_ctx *ctx, int32_t *fused);
 
+/* ---- gradients ------------------------------------------------------------
+ * No counterpart in the reference, which has no adjoint.  Differentiates the
+ * fused step (fluid velocity zero) with respect to the per-cell emissivity eta
+ * and absorption kappa0, for a loss on bilinear samples of the final state.
+ * Requires rt_uses_fused_path == 1. */
+/* Runs nSteps steps like rt_step and keeps the state before steps 0, every,
+ * 2*every, ... on the device for rt_adjoint (ceil(nSteps/every) states). */
+int rt_forward_checkpointed(rt_ctx *ctx, int32_t nSteps, int32_t every);
+/* Bilinear samples of the current intensities at npts points (x, y): out
+ * [npts*nDir] gets I_d at the point for the directions pointing outward
+ * (n_d . (x, y) > 0) and 0 for the others (an observer ring around the
+ * origin). */
+int rt_sample_outgoing(rt_ctx *ctx, int32_t npts, const double *x, const double *y, double *out);
+/* Gradient of a loss L(samples) of the state after the last
+ * rt_forward_checkpointed run: dL_dout [npts*nDir] is dL/d(out) of
+ * rt_sample_outgoing at the same points.  Recomputes each checkpoint segment
+ * and runs the adjoint steps backwards; writes dL/d eta and dL/d kappa0 ([N]
+ * each, zero for halo and excised cells).  The state is left unchanged. */
+int rt_adjoint(rt_ctx *ctx, int32_t npts, const double *x, const double *y, const double *dL_dout,
+               double *grad_eta, double *grad_kappa0);
+
 /* ---- timing helper ------------------------------------------------------- */
 /* Runs rt_step(ctx, nSteps, 0) bracketed by HIP events recorded on the
  * context's own stream and returns the elapsed milliseconds. */
