@@ -51,7 +51,9 @@ template <typename T, int D, bool CURVED>
 struct ASmem {
     static constexpr int RW = 66;
     static constexpr int NC = CURVED ? 26 : 1;
-    T A[3][D + 4][RW]; /* adjoint accumulation, slot = row % 3 */
+    /* adjoint accumulation, slot = row % 3; FP64 in both precisions (LDS
+     * FP32 atomic adds measured about 4x slower here than FP64 ones) */
+    double A[3][D + 4][RW];
     double C[NC][64];
     double X[RT_FUSED_WAVES][4][64];
     float wrow[10][12] __attribute__((aligned(16)));
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
     const int rowB = min(rowA + p.rows_per_block, p.ny - 1);
     if (rowA >= rowB) return;
 
-    for (int e = threadIdx.x; e < 3 * NA; e += 256) (&sm.A[0][0][0])[e] = T(0);
+    for (int e = threadIdx.x; e < 3 * NA; e += 256) (&sm.A[0][0][0])[e] = 0.0;
     if constexpr (CURVED && sizeof(T) == 4) {
         if (threadIdx.x < 120) (&sm.wrow[0][0])[threadIdx.x] = (&dtab0->wrowf[0][0])[threadIdx.x];
     }
@@ -107,16 +109,16 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
     /* adds the finished accumulation row `row` into the global adjoint and
      * clears its slot */
     auto flush = [&](int row) {
-        T *slot = &sm.A[row % 3][0][0];
+        double *slot = &sm.A[row % 3][0][0];
         for (int e = threadIdx.x; e < NA; e += 256) {
             const int r = e / RW, cidx = e - r * RW;
-            const T v = slot[e];
+            const double v = slot[e];
             const int col = col0 + cidx;
-            if (v != T(0) && col < nx) {
+            if (v != 0.0 && col < nx) {
                 const int d = r < D ? r : r - D;
-                unsafeAtomicAdd(p.lam_out + (size_t)d * N + (size_t)row * nx + col, v);
+                unsafeAtomicAdd(p.lam_out + (size_t)d * N + (size_t)row * nx + col, (T)v);
             }
-            slot[e] = T(0);
+            slot[e] = 0.0;
         }
     };
     stage_coef(rowA);
@@ -258,12 +260,12 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
                         const int ca = lane + (G.lo_i ? 0 : 1);
 #pragma unroll
                         for (int m = 0; m < 5; m++) {
-                            T *pa = &sm.A[ra][G.s0 + m][ca];
-                            T *pb = &sm.A[rb][G.s0 + m][ca];
-                            atomicAdd(pa, w00 * G.Cw[m]);
-                            atomicAdd(pa + 1, w10 * G.Cw[m]);
-                            atomicAdd(pb, w01 * G.Cw[m]);
-                            atomicAdd(pb + 1, w11 * G.Cw[m]);
+                            double *pa = &sm.A[ra][G.s0 + m][ca];
+                            double *pb = &sm.A[rb][G.s0 + m][ca];
+                            atomicAdd(pa, (double)(w00 * G.Cw[m]));
+                            atomicAdd(pa + 1, (double)(w10 * G.Cw[m]));
+                            atomicAdd(pb, (double)(w01 * G.Cw[m]));
+                            atomicAdd(pb + 1, (double)(w11 * G.Cw[m]));
                         }
                     }
                 }
@@ -280,10 +282,10 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
                     const T f = (T)jb;
                     const int ra = oj < 0 ? slm : sl0, rb = oj < 0 ? sl0 : slp;
                     const int ca = lane + 1 + oi;
-                    atomicAdd(&sm.A[ra][d][ca], f * (T(1) - fx) * (T(1) - fy));
-                    atomicAdd(&sm.A[ra][d][ca + 1], f * fx * (T(1) - fy));
-                    atomicAdd(&sm.A[rb][d][ca], f * (T(1) - fx) * fy);
-                    atomicAdd(&sm.A[rb][d][ca + 1], f * fx * fy);
+                    atomicAdd(&sm.A[ra][d][ca], (double)(f * (T(1) - fx) * (T(1) - fy)));
+                    atomicAdd(&sm.A[ra][d][ca + 1], (double)(f * fx * (T(1) - fy)));
+                    atomicAdd(&sm.A[rb][d][ca], (double)(f * (T(1) - fx) * fy));
+                    atomicAdd(&sm.A[rb][d][ca + 1], (double)(f * fx * fy));
                 }
             }
         }

@@ -92,10 +92,22 @@ def fields(name, grid, seed=None):
         f["kappa0"] = 2.0 * uniform(seed, N, 10)
         f["kappa1"] = 0.3 * uniform(seed, N, 11)
     elif spec["number"] == 4:
+        # the point where gradients are taken: a strictly positive emissivity,
+        # so no intensity sits on the IntensityAt clamp at 0 (a kink where
+        # central differences and the adjoint legitimately disagree)
         g = gaussian_random_field(seed, grid.nx, grid.ny, 0.5 / grid.dx, 0.5 / grid.dy, 0)
-        f["eta"] = np.maximum(0.0, g)
+        f["eta"] = np.maximum(0.05, 0.5 + 0.25 * g)
         f["kappa0"] = uniform(seed, N, 10)
     return f
+
+
+def target_eta(name, grid, seed=None):
+    """Emissivity eta* of the observer-ring target of the gradient workload:
+    a clipped unit-variance GRF (correlation length 0.5) with its own seed."""
+    spec = WORKLOADS[name]
+    seed = 2000 + spec["number"] if seed is None else seed
+    g = gaussian_random_field(seed, grid.nx, grid.ny, 0.5 / grid.dx, 0.5 / grid.dy, 0)
+    return np.maximum(0.0, g).reshape(-1)
 
 
 def spacetime_kind(spec):

@@ -7,8 +7,10 @@ fields.  A step is one timestep (stream + moments + collide) over the grid.
 
 Prints one JSON line: FP32-mixed cell*direction updates/s as the value, the
 FP64 GPU path, their ratio and the FP32-vs-FP64 rel-L2 of I, E, Fx, Fy after
-the timed steps, the HBM roofline of the fused kernel and the CPU baseline
-(the oracle, OpenMP on the host cores, on a bounded number of steps).
+the timed steps, the HBM roofline of the fused kernel, the adjoint leg
+(observer-ring gradients on Schwarzschild 256x256 x 32, adjoint/forward time
+ratio, central-difference check) and the CPU baseline (the oracle, OpenMP on
+the host cores, on a bounded number of steps).
 """
 import argparse
 import importlib
@@ -93,6 +95,86 @@ def _active(gs, r2):
     return inter & ~((r2 >= 0) & (x * x + y * y <= r2))
 
 
+def adjoint_leg(rt, steps=500, every=50, n_fd=16, h=1e-3):
+    """Gradients of an observer-ring loss: Schwarzschild 256x256 on [-4,4]^2,
+    32 directions, 500 steps with checkpoints every 50, kappa0 ~ U[0,1],
+    eta = max(0.05, 0.5 + 0.25 GRF); target = ring samples of a run with
+    eta* = max(0, GRF) (second seed).
+    FP32-mixed forward and adjoint timed; FP64 adjoint and central
+    differences (h = 1e-3, FP64 forward, 16 seeded active cells) as checks."""
+    name = "schwarzschild_256sq_d32"
+    th = 2 * np.pi * (np.arange(128) + 0.5) / 128
+    px, py = 3.8 * np.cos(th), 3.8 * np.sin(th)
+    out = {"workload": name, "steps": steps, "checkpoint_every": every, "ring_points": 128, "ring_radius": 3.8}
+    grads = {}
+    for prec in (rt.FP32_MIXED, rt.FP64):
+        ctx, meta = rt.workloads.make_context(name, prec)
+        gs = meta["grid"]
+        f = rt.workloads.fields(name, gs)
+        ftarget = dict(f)
+        ftarget["eta"] = rt.workloads.target_eta(name, gs)
+        zero = np.zeros(gs.N * ctx.nDir)
+        ctx.set_fields(**ftarget)
+        ctx.upload_I(zero)
+        ctx.step(steps, want_moments=False)
+        target = ctx.sample_outgoing(px, py)
+        ctx.set_fields(**f)
+        ctx.upload_I(zero)
+        ctx.step(2, want_moments=False)  # warm-up
+        ctx.upload_I(zero)
+        ctx.synchronize()
+        ms_fwd = ctx.time_steps(steps)
+        ctx.upload_I(zero)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.forward_checkpointed(steps, every)
+        t1 = time.perf_counter()
+        obs = ctx.sample_outgoing(px, py)
+        loss = 0.5 * float(np.sum((obs - target) ** 2))
+        t2 = time.perf_counter()
+        ge, gk = ctx.adjoint(px, py, obs - target)
+        t3 = time.perf_counter()
+        key = "fp32" if prec == rt.FP32_MIXED else "fp64"
+        grads[key] = (ge, gk)
+        out[key] = {"loss": loss, "forward_ms": ms_fwd, "forward_checkpointed_ms": 1e3 * (t1 - t0),
+                    "adjoint_ms": 1e3 * (t3 - t2), "adjoint_over_forward": (t3 - t2) * 1e3 / ms_fwd}
+        if prec == rt.FP64:
+            # central differences of the FP64 forward on seeded active cells
+            x, y = gs.coords()
+            i = np.arange(gs.N) % gs.nx
+            j = np.arange(gs.N) // gs.nx
+            ok = (i > 1) & (i < gs.nx - 2) & (j > 1) & (j < gs.ny - 2) & (x * x + y * y > 2.1 ** 2 + 0.5)
+            rng = np.random.default_rng(1004)
+            cells = rng.choice(np.nonzero(ok)[0], n_fd, replace=False)
+            fd_err = {}
+            for hh in (h, 1e-5):
+                for fname, grad in (("eta", ge), ("kappa0", gk)):
+                    fd = []
+                    for c in cells:
+                        vals = []
+                        for sgn in (1, -1):
+                            ff = dict(f)
+                            arr = ff[fname].copy()
+                            arr[c] += sgn * hh
+                            ff[fname] = arr
+                            ctx.set_fields(**ff)
+                            ctx.upload_I(zero)
+                            ctx.step(steps, want_moments=False)
+                            o = ctx.sample_outgoing(px, py)
+                            vals.append(0.5 * float(np.sum((o - target) ** 2)))
+                        fd.append((vals[0] - vals[1]) / (2 * hh))
+                    fd = np.array(fd)
+                    fd_err[f"{fname}_h{hh:g}"] = float(np.linalg.norm(grad[cells] - fd) / np.linalg.norm(fd))
+            ctx.set_fields(**f)
+            out["fd_rel_err_fp64"] = fd_err
+            out["fd_cells"] = int(n_fd)
+        ctx.close()
+    out["rel_l2_grad_fp32_vs_fp64"] = {"eta": rel_l2(grads["fp32"][0], grads["fp64"][0]),
+                                       "kappa0": rel_l2(grads["fp32"][1], grads["fp64"][1])}
+    out["adjoint_over_forward"] = out["fp32"]["adjoint_over_forward"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +184,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (HBM counter calibration)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--ref-steps", type=int, default=200,
                     help="steps of the FP64 reference-order comparison leg (0 skips it)")
     args = ap.parse_args()
@@ -226,6 +309,9 @@ def main():
                 "fp64_fused": {"I": rel_l2(J64, Iref), "E": rel_l2(Eb, Er), "Fx": rel_l2(Fxb, Fxr), "Fy": rel_l2(Fyb, Fyr)}}
     c32.close()
 
+    if not args.no_adjoint:
+        print("[bench] adjoint leg", file=sys.stderr, flush=True)
+        result["adjoint"] = adjoint_leg(rt)
     if not args.no_cpu:
         print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(rt, WORKLOAD)
