@@ -86,6 +86,14 @@ def lib():
     L.rt_forward_checkpointed.argtypes = [vp, i32, i32]
     L.rt_sample_outgoing.argtypes = [vp, i32, _dp, _dp, _dp]
     L.rt_adjoint.argtypes = [vp, i32, _dp, _dp, _dp, _dp, _dp]
+    L.rt_batch_create.argtypes = [C.POINTER(vp), i32, C.POINTER(vp)]
+    L.rt_batch_destroy.argtypes = [vp]
+    L.rt_batch_last_error.restype = C.c_char_p
+    L.rt_batch_last_error.argtypes = [vp]
+    L.rt_batch_step.argtypes = [vp, i32]
+    L.rt_batch_time_steps.argtypes = [vp, i32, C.POINTER(C.c_float)]
+    L.rt_batch_autotune.argtypes = [vp, i32, vp]
+    L.rt_batch_groups.argtypes = [vp, C.POINTER(i32)]
     L.rt_build_stencil.argtypes = [i32, _dp, _dp, _dp, _dp, vp, vp]
     L.rt_build_metric.argtypes = [i32, d, d, i32, i32, d, d, d, d, vp, vp, vp, vp, vp, vp]
     _lib = L
@@ -273,6 +281,55 @@ class Context:
         ms = C.c_float()
         self._c(lib().rt_time_steps(self.h, n, C.byref(ms)))
         return ms.value
+
+
+class Batch:
+    """Many Contexts advanced together (include/rt2d.h: rt_batch_*): one
+    grouped launch per (precision, nDir, streaming) class per step, two steps
+    captured in a HIP graph.  The Contexts must stay alive while it exists."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        self.h = C.c_void_p()
+        rc = lib().rt_batch_create(arr, len(self.contexts), C.byref(self.h))
+        if rc != RT_OK:
+            msg = lib().rt_last_error(self.contexts[0].h)
+            raise RtError(rc, msg.decode() if msg else "")
+
+    def _c(self, rc):
+        if rc != RT_OK:
+            raise RtError(rc, lib().rt_batch_last_error(self.h).decode())
+
+    def groups(self):
+        n = C.c_int32()
+        self._c(lib().rt_batch_groups(self.h, C.byref(n)))
+        return n.value
+
+    def step(self, n):
+        self._c(lib().rt_batch_step(self.h, n))
+
+    def time_steps(self, n):
+        ms = C.c_float()
+        self._c(lib().rt_batch_time_steps(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def autotune(self, trial_steps=2):
+        """Rows per workgroup chosen per class (0 = default rule)."""
+        chosen = np.zeros(self.groups(), dtype=np.int32)
+        self._c(lib().rt_batch_autotune(self.h, trial_steps, chosen.ctypes.data))
+        return chosen
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 from . import workloads  # noqa: E402,F401

@@ -581,7 +581,7 @@ static int ensure_unscaled(rt_ctx *c)
 
 /* State convention of the fused kernel: curved state as alpha^-3 I, and the
  * halo and excised cells equal in both buffers. */
-static int prepare_fused(rt_ctx *c)
+int prepare_fused(rt_ctx *c)
 {
     if (c->desc.streaming == RT_CURVED_FIXED && !c->scaled) {
         TRY(launch_scale_state(c, c->d_I[c->cur], false));

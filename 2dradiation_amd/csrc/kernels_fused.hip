@@ -35,6 +35,8 @@
  * ray, so the angular weight rows sum exactly to 1 and 0 in FP32 and the
  * ray factor s enters as s - 1.
  */
+#include <string>
+#include <vector>
 #include "fused_common.h"
 
 namespace {
@@ -156,352 +158,46 @@ constexpr int fused_min_blocks()
     return sizeof(Smem<T, D, CURVED>) <= 80 * 1024 ? 2 : 1;
 }
 
-template <typename T, int D, bool CURVED, bool REC>
-__global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fused(KP<T> p)
+/* XCD-aware tile order: consecutive workgroups are dealt round-robin over
+ * the 8 XCDs, so each XCD gets a contiguous run of tiles (vertically adjacent
+ * chunks of a strip share their halo rows in that XCD's L2). */
+__device__ __forceinline__ int xcd_tile(int bid, int nblocks)
 {
-    constexpr int DW = D / RT_FUSED_WAVES;
-    constexpr int RW = Smem<T, D, CURVED>::RW;
-    constexpr int NC = Smem<T, D, CURVED>::NC;
-    static_assert(DW <= 32, "exclusion bits of one wavefront must fit 32 bits");
-    __shared__ Smem<T, D, CURVED> sm;
-
-    const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    CDirTables *dtab0 = (CDirTables *)p.dt;
-    const long N = p.N;
-    const int nx = p.nx;
-
-    /* XCD-aware tile order: consecutive workgroups are dealt round-robin over
-     * the 8 XCDs, so give each XCD a contiguous run of tiles (vertically
-     * adjacent chunks of a strip share their halo rows in that XCD's L2). */
-    int bid = blockIdx.x;
-    int tile = bid;
-    if ((p.nblocks & 7) == 0) tile = (bid & 7) * (p.nblocks >> 3) + (bid >> 3);
-    const int strip = tile / p.chunks;
-    const int chunk = tile % p.chunks;
-    const int col0 = 64 * strip;                /* grid column of LDS column 0 */
-    const int gi = col0 + 1 + lane;             /* this lane's cell column */
-    const bool inside = gi <= nx - 2;
-    const int rowA = 1 + chunk * p.rows_per_block;
-    const int rowB = min(rowA + p.rows_per_block, p.ny - 1); /* exclusive */
-    if (rowA >= rowB) return;
-    const int ncols = min(RW, nx - col0);       /* valid LDS columns */
-
-    if constexpr (CURVED && sizeof(T) == 4) {
-        if (threadIdx.x < 120) (&sm.wrow[0][0])[threadIdx.x] = (&dtab0->wrowf[0][0])[threadIdx.x];
-    }
-
-    /* ---- asynchronous staging (direct global -> LDS) ---- */
-    auto stage_row = [&](int row) {
-        const int slot = row % 3;
-        const int ndw = (int)(ncols * sizeof(T) / 4);
-        for (int r = wv; r < D + 4; r += RT_FUSED_WAVES) {
-            const int d = r < D ? r : r - D;
-            const T *src = p.Iin + (size_t)d * N + (size_t)row * nx + col0;
-            lds_dma_dwords(src, &sm.I[slot][r][0], ndw, lane);
-        }
-    };
-    auto stage_coef = [&](int row) {
-        if constexpr (CURVED) {
-            const int n = (int)(min(64, nx - 1 - (col0 + 1)) * 2); /* interior cells of the strip */
-            const size_t off = (size_t)row * nx + col0 + 1;
-            for (int e = wv; e < NC; e += RT_FUSED_WAVES) {
-                const void *src = e < 25 ? (const void *)(p.coef + (size_t)e * N + off) : (const void *)(p.excl + off);
-                lds_dma_dwords(src, &sm.C[e][0], n, lane);
-            }
-        }
-    };
-
-    /* prologue: rows rowA-1, rowA, rowA+1 and the coefficients of rowA */
-    for (int r = rowA - 1; r <= rowA + 1; r++) stage_row(r);
-    stage_coef(rowA);
-
-    unsigned long long it_sum = 0, it_max = 0;
-    const double dtime = p.dtime;
-    int sl = (rowA - 1) % 3; /* LDS slot of row j-1 */
-
-    for (int j = rowA; j < rowB; j++) {
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads(); /* slots j-1..j+1 and the coefficients of row j are in LDS */
-        const bool more = (j + 1 < rowB);
-        const int slm = sl, sl0 = sl == 2 ? 0 : sl + 1, slp = sl0 == 2 ? 0 : sl0 + 1;
-        sl = sl0;
-
-        const unsigned ij = (unsigned)j * (unsigned)nx + (unsigned)gi;
-        const uint8_t mk = inside ? p.mask[ij] : 0;
-        const bool active = (mk & MASK_ACTIVE) != 0;
-        /* per-cell inputs, consumed by the collide below */
-        T f_eta = 0, f_k0 = 0, f_k1 = 0, f_kA = 0, f_ux = 0, f_uy = 0, f_adt = 0, f_ia3 = 1;
-        if (active) {
-            f_eta = (p.cell + (size_t)CF_ETA * N)[ij];
-            f_k0 = (p.cell + (size_t)CF_K0 * N)[ij];
-            f_k1 = (p.cell + (size_t)CF_K1 * N)[ij];
-            f_kA = (p.cell + (size_t)CF_KA * N)[ij];
-            f_ux = (p.cell + (size_t)CF_UX * N)[ij];
-            f_uy = (p.cell + (size_t)CF_UY * N)[ij];
-            f_adt = (p.cell + (size_t)CF_ADT * N)[ij];
-            if (CURVED) f_ia3 = (p.cell + (size_t)CF_IA3 * N)[ij];
-        }
-        /* An opaque copy of the table pointer per row keeps the per-direction
-         * constants from being hoisted out of the row loop. */
-        CDirTables *dtab = dtab0;
-        asm volatile("" : "+s"(dtab));
-
-        /* this lane's column in the three row slots */
-        const T *colm = &sm.I[slm][0][lane];
-        const T *col0p = &sm.I[sl0][0][lane];
-        const T *colp = &sm.I[slp][0][lane];
-
-        /* ---- stream ---- */
-        T sv[DW];
-        double acc[6] = {0, 0, 0, 0, 0, 0};
-        unsigned long long mbits = 0; /* REC: bit 4k+c set when corner c of direction k is not clamped */
-        if constexpr (CURVED) {
-            /* geodesic Fourier coefficients of this cell: S as S - 1, X and Y
-             * relative to the cell centre in cell units */
-            T cS[5], cX[5], cY[5], cU[5], cV[5];
-            cell_series<T>(sm.C, lane, p.startx + gi * p.dx, p.starty + j * p.dy, 1.0 / p.dx, 1.0 / p.dy, cS, cX, cY,
-                           cU, cV);
-            /* directions of this wavefront whose departure point is excised */
-            const uint32_t exw = (uint32_t)(__double_as_longlong(sm.C[25][lane]) >> (wv * DW));
-            const T angu = (sizeof(T) == 4) ? (T)dtab->ang_uf : (T)dtab->ang_u;
-            const int quarter = dtab->quarter, nGrid = dtab->nGrid;
-            const T ddmax = T(nGrid) - T(1e-8);
-            /* Software pipeline over the directions of this wavefront: the
-             * LDS taps of direction k+1 are in flight while direction k is
-             * combined.  prep() computes the departure point, angular window
-             * and weights and issues the 20 tap loads; fin() clamps,
-             * interpolates and applies the ray factor.  Everything is
-             * computed for every lane (no branches); lanes that do not gather
-             * select 0.  The clamps keep every LDS index in range for any
-             * input, NaN included. */
-            struct Prep {
-                T v[5][4];
-                T Cw[5];
-                T fx, fy, s;
-                bool go;
-            };
-            auto prep = [&](int k, Prep &P) {
-                const int d = wv * DW + k;
-                DirGeom<T> G;
-                curved_geometry<T, D>(cS, cX, cY, cU, cV, dtab, dtab0, &sm.wrow[0][0], d, angu, quarter, nGrid, ddmax,
-                                      G);
-                P.s = G.s;
-                P.go = active && !((exw >> k) & 1u);
-                P.fx = G.fx;
-                P.fy = G.fy;
-#pragma unroll
-                for (int m = 0; m < 5; m++) P.Cw[m] = G.Cw[m];
-                const T *ra = G.lo_j ? colm : col0p;
-                const T *rb = G.lo_j ? col0p : colp;
-                const int off = (int)__umul24((unsigned)G.s0, (unsigned)RW) + (G.lo_i ? 0 : 1);
-                const T *pa = ra + off;
-                const T *pb = rb + off;
-#pragma unroll
-                for (int m = 0; m < 5; m++) {
-                    P.v[m][0] = pa[m * RW];
-                    P.v[m][1] = pa[m * RW + 1];
-                    P.v[m][2] = pb[m * RW];
-                    P.v[m][3] = pb[m * RW + 1];
-                }
-            };
-            auto fin = [&](int k, const Prep &P) {
-                const int d = wv * DW + k;
-                T v00 = 0, v10 = 0, v01 = 0, v11 = 0;
-#pragma unroll
-                for (int m = 0; m < 5; m++) {
-                    v00 += P.Cw[m] * P.v[m][0];
-                    v10 += P.Cw[m] * P.v[m][1];
-                    v01 += P.Cw[m] * P.v[m][2];
-                    v11 += P.Cw[m] * P.v[m][3];
-                }
-                /* state is alpha^-3 I: the (alpha/alpha_k)^3 factors cancel.  s is
-                 * carried as s - 1, so s^3 x = x + (s^3 - 1) x rounds once, at the end */
-                const T bil = rtm::bilinear<T>(P.fx, P.fy, tmax0(v00), tmax0(v01), tmax0(v10), tmax0(v11));
-                if constexpr (REC) {
-                    const unsigned bits = (unsigned)(v00 > T(0)) | ((unsigned)(v01 > T(0)) << 1) |
-                                          ((unsigned)(v10 > T(0)) << 2) | ((unsigned)(v11 > T(0)) << 3);
-                    mbits |= (unsigned long long)bits << (4 * k);
-                }
-                const T s3m1 = P.s * (T(3) + P.s * (T(3) + P.s));
-                const T val = P.go ? bil + bil * s3m1 : T(0);
-                sv[k] = val;
-                /* materialise this direction's value here: otherwise the
-                 * compiler sinks every gather to the end of the row and
-                 * spills the taps of all directions */
-                asm volatile("" ::"v"(val));
-                const double vd = (double)val;
-#pragma unroll
-                for (int X = 0; X < 6; X++) acc[X] += dtab->w6[d][X] * vd;
-            };
-            Prep P0, P1;
-            prep(0, P0);
-#pragma unroll
-            for (int k = 0; k < DW; k += 2) {
-                if (k + 1 < DW) prep(k + 1, P1);
-                fin(k, P0);
-                if (k + 1 < DW) {
-                    if (k + 2 < DW) prep(k + 2, P0);
-                    fin(k + 1, P1);
-                }
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < DW; k++) {
-                const int d = wv * DW + k;
-                T fx, fy;
-                int oi, oj;
-                if constexpr (sizeof(T) == 4) {
-                    oi = dtab->flat_di[d];
-                    oj = dtab->flat_dj[d];
-                    fx = dtab->flat_fxf[d];
-                    fy = dtab->flat_fyf[d];
-                } else {
-                    /* reference arithmetic of StreamFlatFixed for this cell */
-                    double xT = p.startx + gi * p.dx, yT = p.starty + j * p.dy;
-                    xT -= dtab->cx[d] * dtime;
-                    yT -= dtab->cy[d] * dtime;
-                    double iT = (xT - p.startx) / p.dx, jT = (yT - p.starty) / p.dy;
-                    double i0 = floor(iT), j0 = floor(jT);
-                    oi = (int)i0 - gi;
-                    oj = (int)j0 - j;
-                    oi = oi < -1 ? -1 : (oi > 0 ? 0 : oi);
-                    oj = oj < -1 ? -1 : (oj > 0 ? 0 : oj);
-                    fx = (T)(iT - (double)(gi + oi));
-                    fy = (T)(jT - (double)(j + oj));
-                }
-                const T *ra = oj < 0 ? colm : col0p;
-                const T *rb = oj < 0 ? col0p : colp;
-                const int off = d * RW + 1 + oi;
-                const T *pa = ra + off;
-                const T *pb = rb + off;
-                T val = rtm::bilinear<T>(fx, fy, pa[0], pb[0], pa[1], pb[1]);
-                val = active ? val : T(0);
-                sv[k] = val;
-                const double vd = (double)val;
-#pragma unroll
-                for (int X = 0; X < 6; X++) acc[X] += dtab->w6[d][X] * vd;
-            }
-        }
-#pragma unroll
-        for (int X = 0; X < 6; X++) sm.X[wv][X][lane] = acc[X];
-        __syncthreads(); /* partial moments visible; every wave is done with slot (j-1)%3 and C */
-        if (more) {
-            stage_row(j + 2);
-            stage_coef(j + 1);
-        }
-
-        /* moments of the streamed intensities (true units: alpha^3 x state) */
-        const double a3 = CURVED ? 1.0 / (double)f_ia3 : 1.0;
-        double mI[6];
-#pragma unroll
-        for (int X = 0; X < 6; X++)
-            mI[X] = a3 * (((sm.X[0][X][lane] + sm.X[1][X][lane]) + sm.X[2][X][lane]) + sm.X[3][X][lane]);
-
-        /* ---- collide ---- */
-        const double eta = (double)f_eta, k0 = (double)f_k0, k1 = (double)f_k1, kA = (double)f_kA;
-        const double u1 = (double)f_ux, u2 = (double)f_uy, adt = (double)f_adt;
-        const bool moving = active && (u1 != 0.0 || u2 != 0.0);
-        double mOut[6];
-        int cycles = 0;
-        if (!__any(moving)) {
-            /* zero fluid velocity: lambda iteration on the moments */
-            const double q = 1.0 / (1.0 + adt * (kA + k0));
-            const double cc = 3.0 * k1 * adt * q;
-            double prev[6];
-#pragma unroll
-            for (int X = 0; X < 6; X++) prev[X] = mI[X];
-            double b = 0, fFx = 0, fFy = 0;
-            double ss = 1;
-            if (active) {
-                /* stop when sqrt(sum ((prev - new) / new)^2) <= 1e-4; the
-                 * quotients use a refined reciprocal, and new == 0 keeps the
-                 * IEEE quotient (inf, or NaN for 0/0, which stops the loop) */
-                while (ss > 1e-8 && cycles < 100) {
-                    cycles++;
-                    b = adt * q * (eta + k0 * prev[0]);
-                    fFx = prev[1];
-                    fFy = prev[2];
-                    double acc2 = 0;
-#pragma unroll
-                    for (int X = 0; X < 6; X++) {
-                        const double nw = q * mI[X] + b * dtab->S[0][X] + cc * (fFx * dtab->S[1][X] + fFy * dtab->S[2][X]);
-                        const double r0 = __builtin_amdgcn_rcp(nw);
-                        const double r1 = fma(r0, fma(-nw, r0, 1.0), r0);
-                        const double r = (prev[X] - nw) * (nw == 0.0 ? r0 : r1);
-                        acc2 += r * r;
-                        prev[X] = nw;
-                    }
-                    ss = acc2;
-                }
-            }
-#pragma unroll
-            for (int X = 0; X < 6; X++) mOut[X] = prev[X];
-            /* Inew = J + (q-1) J + b + c n.F: the small factor q-1 carries the
-             * rounding, which keeps FP32 storage unbiased; b and c are scaled
-             * to the state's alpha^-3 units */
-            const double is = CURVED ? (double)f_ia3 : 1.0;
-            const T qm1 = (T)(q - 1.0), bT = (T)(b * is), cxT = (T)(cc * fFx * is), cyT = (T)(cc * fFy * is);
-#pragma unroll
-            for (int k = 0; k < DW; k++) {
-                const int d = wv * DW + k;
-                const T nxd = (sizeof(T) == 4) ? (T)dtab->cxf[d] : (T)dtab->cx[d];
-                const T nyd = (sizeof(T) == 4) ? (T)dtab->cyf[d] : (T)dtab->cy[d];
-                sv[k] = sv[k] + (qm1 * sv[k] + (bT + (cxT * nxd + cyT * nyd)));
-            }
-        } else {
-            if (CURVED) {
-#pragma unroll
-                for (int k = 0; k < DW; k++) sv[k] = (T)((double)sv[k] * a3);
-            }
-            collide_moving<T, DW>(sv, mOut, cycles, mI, active, eta, k0, k1, kA, u1, u2, adt, wv, lane, dtab,
-                                  sm.X);
-            if (CURVED) {
-#pragma unroll
-                for (int k = 0; k < DW; k++) sv[k] = (T)((double)sv[k] * (double)f_ia3);
-            }
-        }
-
-        /* ---- write back ---- */
-        if constexpr (REC) {
-            if (active) {
-                if (wv == 0) {
-#pragma unroll
-                    for (int X = 0; X < 6; X++) (p.rec_m + (size_t)X * N)[ij] = mI[X];
-                    p.rec_k[ij] = (uint8_t)cycles;
-                }
-                if (CURVED) (p.rec_mask + (size_t)wv * N)[ij] = mbits;
-            }
-        }
-        if (active) {
-#pragma unroll
-            for (int k = 0; k < DW; k++) (p.Iout + (size_t)(wv * DW + k) * N)[ij] = sv[k];
-            if (wv == 0) {
-                it_sum += (unsigned long long)cycles;
-                it_max = it_max > (unsigned long long)cycles ? it_max : (unsigned long long)cycles;
-                if (p.want_moments) {
-#pragma unroll
-                    for (int X = 0; X < 6; X++) (p.mom + (size_t)X * N)[ij] = mOut[X];
-                }
-            }
-        }
-    }
-
-    if (wv == 0) {
-        for (int off = 32; off > 0; off >>= 1) {
-            it_sum += __shfl_xor(it_sum, off);
-            unsigned long long o = __shfl_xor(it_max, off);
-            it_max = it_max > o ? it_max : o;
-        }
-        if (lane == 0) {
-            atomicAdd(&p.stats[0], it_sum);
-            atomicMax(&p.stats[1], it_max);
-        }
-    }
+    return (nblocks & 7) == 0 ? (bid & 7) * (nblocks >> 3) + (bid >> 3) : bid;
 }
 
 template <typename T, int D, bool CURVED, bool REC>
-int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments, const RecordPtrs *rec)
+__global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fused(KP<T> p)
+{
+    const int tile = xcd_tile(blockIdx.x, p.nblocks);
+    const T *Iin = p.Iin;
+    T *Iout = p.Iout;
+#include "fused_tile_body.inc"
+}
+
+/* Many independent problems of one (precision, nDir, streaming) class in one
+ * launch: workgroups [start[i], start[i+1]) run the tiles of problem i.  The
+ * problem table holds each problem's step with `parity` 0; odd parity swaps
+ * the two state buffers. */
+template <typename T, int D, bool CURVED>
+__global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fused_batch(
+    const KP<T> *__restrict__ probs, const int *__restrict__ start, int nprob, int parity)
+{
+    constexpr bool REC = false;
+    int i = 0;
+    while (i + 1 < nprob && start[i + 1] <= (int)blockIdx.x) i++;
+    i = __builtin_amdgcn_readfirstlane(i);
+    const KP<T> &p = probs[i];
+    const int tile = xcd_tile((int)blockIdx.x - start[i], p.nblocks);
+    const T *Iin = parity ? (const T *)p.Iout : p.Iin;
+    T *Iout = parity ? const_cast<T *>(p.Iin) : p.Iout;
+#include "fused_tile_body.inc"
+}
+
+/* Kernel parameters of one fused step of context c; H = rows per workgroup
+ * (0: about 1024 workgroups with at least 4 rows each). */
+template <typename T>
+KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, const RecordPtrs *rec, int H = 0)
 {
     const rt_desc &d = c->desc;
     KP<T> p;
@@ -519,11 +215,12 @@ int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments, const Re
     p.N = (long)d.nx * d.ny;
     p.strips = (d.nx - 2 + 63) / 64;
     const int rows = d.ny - 2;
-    /* aim for about 1024 workgroups (4 per CU) with at least 4 rows each */
-    int chunks = (1024 + p.strips - 1) / p.strips;
-    int H = (rows + chunks - 1) / chunks;
-    H = H < 4 ? 4 : (H > 64 ? 64 : H);
-    chunks = (rows + H - 1) / H;
+    if (H <= 0) {
+        const int target = (1024 + p.strips - 1) / p.strips;
+        H = (rows + target - 1) / target;
+        H = H < 4 ? 4 : (H > 64 ? 64 : H);
+    }
+    const int chunks = (rows + H - 1) / H;
     p.chunks = chunks;
     p.rows_per_block = H;
     p.nblocks = p.strips * chunks;
@@ -537,6 +234,13 @@ int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments, const Re
     p.rec_m = rec ? rec->m : nullptr;
     p.rec_k = rec ? rec->k : nullptr;
     p.rec_mask = rec ? rec->mask : nullptr;
+    return p;
+}
+
+template <typename T, int D, bool CURVED, bool REC>
+int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments, const RecordPtrs *rec)
+{
+    KP<T> p = make_kp<T>(c, in, out, want_moments, rec);
     k_fused<T, D, CURVED, REC><<<p.nblocks, 256, 0, c->stream>>>(p);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
@@ -592,3 +296,312 @@ int launch_fused_step_rec(rt_ctx *c, const void *in, void *out, const RecordPtrs
     return curved ? dispatch_d<double, true, true>(c, in, out, false, &rec)
                   : dispatch_d<double, false, true>(c, in, out, false, &rec);
 }
+
+/* ---- batches of independent problems ------------------------------------ */
+
+struct rt_batch {
+    std::vector<rt_ctx *> ctx;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    struct Group {
+        bool fp32 = false, curved = false;
+        int D = 0;
+        std::vector<int> members;
+        std::vector<int> H;        /* rows per workgroup of each member (0: default) */
+        void *d_kp = nullptr;      /* KP<T> table */
+        int *d_start = nullptr;    /* first workgroup of each member, then the total */
+        int nblocks = 0;
+    };
+    std::vector<Group> groups;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+int bfail(rt_batch *b, int code, const std::string &msg)
+{
+    if (b) b->err = msg;
+    return code;
+}
+#define BHIP(b, call)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (call);                                                            \
+        if (_e != hipSuccess) return bfail((b), RT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+template <typename T>
+int fill_group_table(rt_batch *b, rt_batch::Group &g)
+{
+    std::vector<KP<T>> kp;
+    std::vector<int> start;
+    int total = 0;
+    for (size_t m = 0; m < g.members.size(); m++) {
+        rt_ctx *c = b->ctx[g.members[m]];
+        kp.push_back(make_kp<T>(c, c->d_I[c->cur], c->d_I[1 - c->cur], false, nullptr, g.H[m]));
+        start.push_back(total);
+        total += kp.back().nblocks;
+    }
+    start.push_back(total);
+    g.nblocks = total;
+    if (!g.d_kp) BHIP(b, hipMalloc(&g.d_kp, kp.size() * sizeof(KP<T>)));
+    if (!g.d_start) BHIP(b, hipMalloc((void **)&g.d_start, start.size() * sizeof(int)));
+    BHIP(b, hipMemcpyAsync(g.d_kp, kp.data(), kp.size() * sizeof(KP<T>), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(g.d_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return RT_OK;
+}
+
+template <typename T, bool CURVED>
+int launch_group_t(rt_batch *b, const rt_batch::Group &g, int parity)
+{
+    const KP<T> *kp = (const KP<T> *)g.d_kp;
+    const int n = (int)g.members.size();
+    switch (g.D) {
+    case 16: k_fused_batch<T, 16, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    case 20: k_fused_batch<T, 20, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    case 24: k_fused_batch<T, 24, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    case 32: k_fused_batch<T, 32, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    case 48: k_fused_batch<T, 48, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    case 64: k_fused_batch<T, 64, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    default: return bfail(b, RT_ERR_UNSUPPORTED, "batch: unsupported nDir");
+    }
+    BHIP(b, hipGetLastError());
+    return RT_OK;
+}
+
+int launch_group(rt_batch *b, const rt_batch::Group &g, int parity)
+{
+    if (g.fp32) return g.curved ? launch_group_t<float, true>(b, g, parity) : launch_group_t<float, false>(b, g, parity);
+    return g.curved ? launch_group_t<double, true>(b, g, parity) : launch_group_t<double, false>(b, g, parity);
+}
+
+/* One step of every problem (one launch per group). */
+int launch_all(rt_batch *b, int parity)
+{
+    for (const auto &g : b->groups)
+        if (int r = launch_group(b, g, parity)) return r;
+    return RT_OK;
+}
+
+/* Brings every member to the fused state convention and refreshes the
+ * problem tables (their buffers follow each context's current state). */
+int batch_prepare(rt_batch *b)
+{
+    for (rt_ctx *c : b->ctx) {
+        if (int r = prepare_fused(c)) return bfail(b, r, std::string("batch member: ") + c->err);
+        c->ck_valid = false;
+        BHIP(b, hipStreamSynchronize(c->stream));
+    }
+    for (auto &g : b->groups)
+        if (int r = g.fp32 ? fill_group_table<float>(b, g) : fill_group_table<double>(b, g)) return r;
+    return RT_OK;
+}
+
+void drop_graph(rt_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+/* A graph of two steps (parity 0 then 1): replays leave every state in its
+ * current buffer. */
+int capture_graph(rt_batch *b)
+{
+    drop_graph(b);
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int r = launch_all(b, 0);
+    if (r == RT_OK) r = launch_all(b, 1);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (r != RT_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return bfail(b, RT_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    b->graph = g;
+    BHIP(b, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return RT_OK;
+}
+
+/* nSteps steps of every problem: graph replays for the pairs, one direct
+ * step for an odd remainder (after which the members' current buffers flip). */
+int batch_run(rt_batch *b, int nSteps)
+{
+    for (int k = 0; k < nSteps / 2; k++) BHIP(b, hipGraphLaunch(b->exec, b->stream));
+    if (nSteps & 1) {
+        if (int r = launch_all(b, 0)) return r;
+        for (rt_ctx *c : b->ctx) c->cur = 1 - c->cur;
+        BHIP(b, hipStreamSynchronize(b->stream));
+        for (auto &g : b->groups)
+            if (int r = g.fp32 ? fill_group_table<float>(b, g) : fill_group_table<double>(b, g)) return r;
+    }
+    return RT_OK;
+}
+} // namespace
+
+extern "C" {
+
+int rt_batch_create(rt_ctx *const *ctxs, int32_t n, rt_batch **out)
+{
+    if (!out) return RT_ERR_ARG;
+    *out = nullptr;
+    if (!ctxs || n < 1) return RT_ERR_ARG;
+    rt_batch *b = new rt_batch();
+    auto fail = [&](int code, const std::string &msg) {
+        if (ctxs[0]) rt_fail(ctxs[0], code, "rt_batch_create: " + msg);
+        rt_batch_destroy(b);
+        return code;
+    };
+    for (int i = 0; i < n; i++) {
+        rt_ctx *c = ctxs[i];
+        if (!c) return fail(RT_ERR_ARG, "null context");
+        if (i == 0) b->device = c->desc.device;
+        if (c->desc.device != b->device) return fail(RT_ERR_ARG, "contexts on different devices");
+        if (!c->has_stencil || !c->has_fields || (c->desc.streaming == RT_CURVED_FIXED && !c->has_tables))
+            return fail(RT_ERR_STATE, "context " + std::to_string(i) + " is not ready to step");
+        if (!fused_supported(c)) return fail(RT_ERR_UNSUPPORTED, "context " + std::to_string(i) + " is not covered by the fused step");
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == c) return fail(RT_ERR_ARG, "a context appears twice");
+        b->ctx.push_back(c);
+        const bool curved = c->desc.streaming == RT_CURVED_FIXED;
+        rt_batch::Group *g = nullptr;
+        for (auto &gg : b->groups)
+            if (gg.fp32 == c->fp32 && gg.curved == curved && gg.D == c->D) g = &gg;
+        if (!g) {
+            b->groups.emplace_back();
+            g = &b->groups.back();
+            g->fp32 = c->fp32;
+            g->curved = curved;
+            g->D = c->D;
+        }
+        g->members.push_back(i);
+        g->H.push_back(0);
+    }
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+        return fail(RT_ERR_HIP, "stream creation failed");
+    if (int r = batch_prepare(b)) return fail(r, b->err);
+    if (int r = capture_graph(b)) return fail(r, b->err);
+    *out = b;
+    return RT_OK;
+}
+
+int rt_batch_destroy(rt_batch *b)
+{
+    if (!b) return RT_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (auto &g : b->groups) {
+        if (g.d_kp) (void)hipFree(g.d_kp);
+        if (g.d_start) (void)hipFree(g.d_start);
+    }
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return RT_OK;
+}
+
+const char *rt_batch_last_error(const rt_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int rt_batch_step(rt_batch *b, int32_t nSteps)
+{
+    if (!b) return RT_ERR_ARG;
+    if (nSteps < 0) return bfail(b, RT_ERR_ARG, "nSteps must be >= 0");
+    if (int r = batch_prepare(b)) return r;
+    if (int r = batch_run(b, nSteps)) return r;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return RT_OK;
+}
+
+int rt_batch_time_steps(rt_batch *b, int32_t nSteps, float *ms)
+{
+    if (!b || !ms) return RT_ERR_ARG;
+    if (nSteps < 0) return bfail(b, RT_ERR_ARG, "nSteps must be >= 0");
+    if (int r = batch_prepare(b)) return r;
+    BHIP(b, hipEventRecord(b->ev0, b->stream));
+    if (int r = batch_run(b, nSteps)) return r;
+    BHIP(b, hipEventRecord(b->ev1, b->stream));
+    BHIP(b, hipEventSynchronize(b->ev1));
+    BHIP(b, hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return RT_OK;
+}
+
+int rt_batch_autotune(rt_batch *b, int32_t trial_steps, int32_t *chosen)
+{
+    if (!b) return RT_ERR_ARG;
+    if (trial_steps < 1) return bfail(b, RT_ERR_ARG, "trial_steps must be >= 1");
+    if (int r = batch_prepare(b)) return r;
+    /* the trials advance the states: keep copies and restore them */
+    std::vector<void *> saved(b->ctx.size(), nullptr);
+    auto release = [&]() {
+        for (void *p : saved)
+            if (p) (void)hipFree(p);
+    };
+    for (size_t i = 0; i < b->ctx.size(); i++) {
+        rt_ctx *c = b->ctx[i];
+        const size_t bytes = (size_t)c->N * c->D * (c->fp32 ? 4 : 8);
+        if (hipMalloc(&saved[i], bytes) != hipSuccess) {
+            release();
+            return bfail(b, RT_ERR_OOM, "autotune: no memory for state copies");
+        }
+        if (hipMemcpyAsync(saved[i], c->d_I[c->cur], bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess) {
+            release();
+            return bfail(b, RT_ERR_HIP, "autotune: state copy failed");
+        }
+    }
+    static const int candidates[] = {0, 4, 8, 16, 32, 64};
+    int gi = 0;
+    for (auto &g : b->groups) {
+        float best = 1e30f;
+        int bestH = 0;
+        for (int H : candidates) {
+            for (auto &h : g.H) h = H;
+            int r = g.fp32 ? fill_group_table<float>(b, g) : fill_group_table<double>(b, g);
+            float ms = 0;
+            /* one untimed step, then an even number of timed ones */
+            if (r == RT_OK) r = launch_group(b, g, 0);
+            if (r == RT_OK && hipEventRecord(b->ev0, b->stream) != hipSuccess) r = RT_ERR_HIP;
+            for (int k = 0; r == RT_OK && k < 2 * trial_steps; k++) r = launch_group(b, g, (k + 1) & 1);
+            if (r == RT_OK && hipEventRecord(b->ev1, b->stream) != hipSuccess) r = RT_ERR_HIP;
+            if (r == RT_OK && hipEventSynchronize(b->ev1) != hipSuccess) r = RT_ERR_HIP;
+            if (r == RT_OK && hipEventElapsedTime(&ms, b->ev0, b->ev1) != hipSuccess) r = RT_ERR_HIP;
+            if (r != RT_OK) {
+                release();
+                return r == RT_ERR_HIP ? bfail(b, r, "autotune: timing failed") : r;
+            }
+            if (ms < best) {
+                best = ms;
+                bestH = H;
+            }
+        }
+        for (auto &h : g.H) h = bestH;
+        if (chosen) chosen[gi] = bestH;
+        gi++;
+    }
+    for (size_t i = 0; i < b->ctx.size(); i++) {
+        rt_ctx *c = b->ctx[i];
+        const size_t bytes = (size_t)c->N * c->D * (c->fp32 ? 4 : 8);
+        if (hipMemcpyAsync(c->d_I[c->cur], saved[i], bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess) {
+            release();
+            return bfail(b, RT_ERR_HIP, "autotune: state restore failed");
+        }
+    }
+    (void)hipStreamSynchronize(b->stream);
+    release();
+    if (int r = batch_prepare(b)) return r;
+    return capture_graph(b);
+}
+
+int rt_batch_groups(const rt_batch *b, int32_t *ngroups)
+{
+    if (!b || !ngroups) return RT_ERR_ARG;
+    *ngroups = (int32_t)b->groups.size();
+    return RT_OK;
+}
+
+} /* extern "C" */

@@ -131,6 +131,8 @@ int launch_ref_moments(rt_ctx *c, const void *in);
 int launch_ref_collide(rt_ctx *c, const void *in, void *out);
 int launch_fused_step(rt_ctx *c, const void *in, void *out, bool want_moments);
 bool fused_supported(const rt_ctx *c);
+/* curved state as alpha^-3 I, fixed cells equal in both buffers */
+extern "C" int prepare_fused(rt_ctx *c);
 int launch_geodesic_tables(rt_ctx *c, const double *d_metric, double tol);
 int launch_departure_bound(rt_ctx *c, double *max_disp);
 int launch_fill(rt_ctx *c, double *p, size_t n, double v);

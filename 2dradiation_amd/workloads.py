@@ -148,3 +148,58 @@ def make_context(name, precision, overrides=None, seed=None, grid_override=None)
     ctx.upload_I(np.zeros(grid.N * spec["nDir"]))
     return ctx, dict(spec=spec, grid=grid, fields=f, metric_block=block, alpha=alpha,
                      tetrad=(t11, t12, t21, t22), excision_r2=r2, kind=kind)
+
+
+def batch_specs(n=64, seed=1005, lo=64, hi=512, dirs=(16, 24, 32, 48, 64)):
+    """The batched workload: n independent problems, nx = ny = round(exp(U[ln lo,
+    ln hi])), nDir uniform from `dirs`, Minkowski on [-1,1]^2 or Schwarzschild
+    (M = 1) on [-6,6]^2 by a fair coin, CFL 0.5.  The Schwarzschild excision
+    radius is max(2.1, 2 + 2 dx): on coarse grids the backward geodesics of
+    cells just outside r = 2.1 reach the horizon's stiff region and leave the
+    cell's neighbourhood, so those cells are excised too."""
+    u = uniform(seed, 3 * n, 20).reshape(3, n)
+    specs = []
+    for k in range(n):
+        size = int(round(np.exp(np.log(lo) + u[0, k] * (np.log(hi) - np.log(lo)))))
+        nd = int(dirs[min(len(dirs) - 1, int(u[1, k] * len(dirs)))])
+        curved = bool(u[2, k] >= 0.5)
+        L = 6.0 if curved else 1.0
+        dx = 2 * L / (size - 1)
+        specs.append(dict(index=k, seed=seed * 1000 + k, size=size, nDir=nd, L=L,
+                          spacetime="schwarzschild" if curved else "minkowski",
+                          excision=max(2.1, 2.0 + 2.0 * dx) if curved else None))
+    return specs
+
+
+def batch_fields(spec, grid):
+    """eta = max(0, GRF) with correlation length L/8, kappa0 ~ lognormal(0, 0.5),
+    kappa1 ~ U[0, 0.5] (seeded per problem)."""
+    N = grid.N
+    corr = spec["L"] / 8.0
+    g = gaussian_random_field(spec["seed"], grid.nx, grid.ny, corr / grid.dx, corr / grid.dy, 0)
+    f = {k: np.zeros(N) for k in ("eta", "kappa0", "kappa1", "kappaA", "ux", "uy")}
+    f["eta"] = np.maximum(0.0, g).reshape(-1)
+    f["kappa0"] = np.exp(0.5 * normal(spec["seed"], N, 1))
+    f["kappa1"] = 0.5 * uniform(spec["seed"], N, 2)
+    return f
+
+
+def make_batch_member(spec, precision):
+    """A ready-to-step Context for one problem of batch_specs."""
+    from . import (Context, GridSpec, build_stencil, build_metric, CURVED_FIXED, FLAT_FIXED,
+                   MINKOWSKI, SCHWARZSCHILD)
+    n, L = spec["size"], spec["L"]
+    grid = GridSpec(n, n, (-L, -L), (L, L), cfl=0.5)
+    curved = spec["spacetime"] == "schwarzschild"
+    r2 = spec.get("excision", 2.1) ** 2 if curved else -1.0
+    ctx = Context(grid, spec["nDir"], CURVED_FIXED if curved else FLAT_FIXED, precision, r2)
+    w, phi, cx, cy, Nc, Wc = build_stencil(spec["nDir"])
+    ctx.set_stencil(w, phi, cx, cy, Nc, Wc)
+    block, alpha, t11, t12, t21, t22 = build_metric(SCHWARZSCHILD if curved else MINKOWSKI, 1.0, 0.0, grid)
+    ctx.set_geometry(alpha, t11, t12, t21, t22)
+    if curved:
+        ctx.set_streaming_stencil(*build_stencil(5, tables=False))
+        ctx.build_fourier_tables(block, 1e-4)
+    ctx.set_fields(**batch_fields(spec, grid))
+    ctx.upload_I(np.zeros(grid.N * spec["nDir"]))
+    return ctx, grid

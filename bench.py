@@ -9,8 +9,9 @@ Prints one JSON line: FP32-mixed cell*direction updates/s as the value, the
 FP64 GPU path, their ratio and the FP32-vs-FP64 rel-L2 of I, E, Fx, Fy after
 the timed steps, the HBM roofline of the fused kernel, the adjoint leg
 (observer-ring gradients on Schwarzschild 256x256 x 32, adjoint/forward time
-ratio, central-difference check) and the CPU baseline (the oracle, OpenMP on
-the host cores, on a bounded number of steps).
+ratio, central-difference check), the batched leg (64 uneven problems in
+graph-replayed grouped launches vs one after another) and the CPU baseline
+(the oracle, OpenMP on the host cores, on a bounded number of steps).
 """
 import argparse
 import importlib
@@ -175,6 +176,47 @@ def adjoint_leg(rt, steps=500, every=50, n_fd=16, h=1e-3):
     return out
 
 
+def batch_leg(rt, steps=200, n=64):
+    """64 independent FP32-mixed problems (nx = ny log-uniform in [64, 512],
+    nDir from {16, 24, 32, 48, 64}, Minkowski or Schwarzschild by a seeded coin),
+    200 steps each: one graph-replayed grouped launch per class per step
+    (autotuned rows per workgroup) against the 64 problems stepped one after
+    another with their own launches.  Both runs must agree bit for bit."""
+    specs = rt.workloads.batch_specs(n)
+    A, B, upd = [], [], 0
+    for sp in specs:
+        a, grid = rt.workloads.make_batch_member(sp, rt.FP32_MIXED)
+        bb, _ = rt.workloads.make_batch_member(sp, rt.FP32_MIXED)
+        A.append(a)
+        B.append(bb)
+        upd += int(np.count_nonzero(_active(grid, sp["excision"] ** 2 if sp["spacetime"] == "schwarzschild" else -1.0))) * sp["nDir"]
+    batch = rt.Batch(A)
+    t0 = time.perf_counter()
+    chosen = batch.autotune(2)
+    t_tune = time.perf_counter() - t0
+    ms_batch = batch.time_steps(steps)
+    # sequential: each problem's own launches, one problem after another
+    for c in B:
+        c.step(1, want_moments=False)  # state convention set-up outside the timing
+    zero_I = None
+    ms_seq = 0.0
+    for c in B:
+        ms_seq += c.time_steps(steps - 1)
+    # compare: A took `steps` steps, B took 1 + (steps - 1)
+    same = all(np.array_equal(a.download_I(), bb.download_I()) for a, bb in zip(A, B))
+    out = {"problems": n, "steps": steps, "classes": int(batch.groups()),
+           "updates_per_step": upd, "batched_ms": ms_batch, "sequential_ms_for_steps_minus_1": ms_seq,
+           "batched_updates_per_s": upd * steps / (ms_batch * 1e-3),
+           "sequential_updates_per_s": upd * (steps - 1) / (ms_seq * 1e-3),
+           "speedup": (upd * steps / ms_batch) / (upd * (steps - 1) / ms_seq),
+           "autotune_s": t_tune, "rows_per_workgroup_chosen": [int(h) for h in chosen],
+           "bitwise_equal_to_sequential": bool(same)}
+    batch.close()
+    for c in A + B:
+        c.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +227,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (HBM counter calibration)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-problems leg")
     ap.add_argument("--ref-steps", type=int, default=200,
                     help="steps of the FP64 reference-order comparison leg (0 skips it)")
     args = ap.parse_args()
@@ -312,6 +355,9 @@ def main():
     if not args.no_adjoint:
         print("[bench] adjoint leg", file=sys.stderr, flush=True)
         result["adjoint"] = adjoint_leg(rt)
+    if not args.no_batch:
+        print("[bench] batch leg", file=sys.stderr, flush=True)
+        result["batch"] = batch_leg(rt)
     if not args.no_cpu:
         print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(rt, WORKLOAD)

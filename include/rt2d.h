@@ -189,6 +189,27 @@ int rt_sample_outgoing(rt_ctx *ctx, int32_t npts, const double *x, const double 
 int rt_adjoint(rt_ctx *ctx, int32_t npts, const double *x, const double *y, const double *dL_dout,
                double *grad_eta, double *grad_kappa0);
 
+/* ---- batches of independent problems ----------------------------------------
+ * No counterpart in the reference (one problem per process there).  Many
+ * contexts, each fully set up (uneven grids, nDir, spacetimes allowed), are
+ * advanced together: per step one grouped launch per (precision, nDir,
+ * streaming) class, two steps captured in a HIP graph and replayed.  Members
+ * must be covered by the fused step (rt_uses_fused_path == 1) and live on one
+ * device; while a batch exists, each member can still be used on its own. */
+typedef struct rt_batch rt_batch;
+int rt_batch_create(rt_ctx *const *ctxs, int32_t n, rt_batch **out);
+int rt_batch_destroy(rt_batch *b);
+const char *rt_batch_last_error(const rt_batch *b);
+/* Advances every member by nSteps steps; returns when the work is done. */
+int rt_batch_step(rt_batch *b, int32_t nSteps);
+/* As rt_batch_step, timed with HIP events on the batch's stream. */
+int rt_batch_time_steps(rt_batch *b, int32_t nSteps, float *ms);
+/* Picks the rows per workgroup of each class by timing 2*trial_steps steps
+ * per candidate (the members' states are restored afterwards); chosen
+ * (optional, one entry per class) gets the choice, 0 = the default rule. */
+int rt_batch_autotune(rt_batch *b, int32_t trial_steps, int32_t *chosen);
+int rt_batch_groups(const rt_batch *b, int32_t *ngroups);
+
 /* ---- timing helper ------------------------------------------------------- */
 /* Runs rt_step(ctx, nSteps, 0) bracketed by HIP events recorded on the
  * context's own stream and returns the elapsed milliseconds. */

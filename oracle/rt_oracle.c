@@ -440,7 +440,11 @@ double or_rk45(const or_grid *g, const double *metric, double bh_r2, double tol,
     double k1n, k2n, k3n, k4n, k5n, k6n;
     double xt[2], vt[2], nt;
     double nu = 1, total = 0, ddt = g->dt;
-    while (total < dt) {
+    /* The reference loops while total < dt; near the horizon the step size
+     * can collapse so that this never ends.  Like the GPU tables, stop after
+     * 100000 step attempts and keep the point reached. */
+    int guard = 0;
+    while (total < dt && guard++ < 100000) {
         ddt = ddt < dt - total ? ddt : dt - total;
         or_rhs(&mv, x[0], x[1], v[0], v[1], nu, k1x, k1v, &k1n);
         for (int d = 0; d < 2; d++) {
