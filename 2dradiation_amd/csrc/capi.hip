@@ -181,6 +181,7 @@ int refresh_departure_bound(rt_ctx *c)
     double m = 0;
     TRY(launch_departure_bound(c, &m));
     c->max_disp = m;
+    TRY(launch_coef_rel(c));
     return RT_OK;
 }
 } // namespace
@@ -256,6 +257,7 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
         (r = dalloc(c, &c->d_basis, D * d.nCoeff * sizeof(double))))
         return fail(r);
     if (d.streaming == RT_CURVED_FIXED && ((r = dalloc(c, &c->d_coef, 5 * (size_t)d.nCoeff * N * sizeof(double))) ||
+                                           (r = dalloc(c, &c->d_coefT, 5 * (size_t)d.nCoeff * N * tsize(c))) ||
                                            (r = dalloc(c, &c->d_excl, N * sizeof(unsigned long long)))))
         return fail(r);
     if ((r = dalloc(c, &c->d_red, sizeof(unsigned long long)))) return fail(r);
@@ -283,7 +285,7 @@ void rt_destroy(rt_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_w); dfree(c->d_phi); dfree(c->d_cx); dfree(c->d_cy); dfree(c->d_Nc); dfree(c->d_Wc);
     dfree(c->d_basis); dfree(c->d_dirt); dfree(c->d_alpha); dfree(c->d_t11); dfree(c->d_t12);
-    dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_excl); dfree(c->d_red);
+    dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_coefT); dfree(c->d_excl); dfree(c->d_red);
     dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
     dfree(c->d_lam[0]); dfree(c->d_lam[1]); dfree(c->d_grad[0]); dfree(c->d_grad[1]); dfree(c->d_pts); dfree(c->d_mask); dfree(c->d_f64);
     dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_mom); dfree(c->d_count);

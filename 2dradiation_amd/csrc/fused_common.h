@@ -128,21 +128,5 @@ __device__ inline void curved_geometry(const T (&cS)[5], const T (&cX)[5], const
     G.s0 = s0 < 0 ? s0 + D : (s0 >= D ? s0 - D : s0);
 }
 
-/* The 25 FP64 Fourier coefficients of a cell from LDS (C[series*5+k][lane]),
- * as S - 1 and X, Y relative to the cell centre in cell units. */
-template <typename T>
-__device__ inline void cell_series(const double (*C)[64], int lane, double xc, double yc, double idx, double idy,
-                                   T (&cS)[5], T (&cX)[5], T (&cY)[5], T (&cU)[5], T (&cV)[5])
-{
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-        cS[k] = (T)(k == 0 ? C[0][lane] - 1.0 : C[k][lane]);
-        cX[k] = (T)((k == 0 ? C[5][lane] - xc : C[5 + k][lane]) * idx);
-        cY[k] = (T)((k == 0 ? C[10][lane] - yc : C[10 + k][lane]) * idy);
-        cU[k] = (T)C[15 + k][lane];
-        cV[k] = (T)C[20 + k][lane];
-    }
-}
-
 } // namespace rtf
 #endif

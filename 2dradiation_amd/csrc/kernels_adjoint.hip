@@ -37,7 +37,7 @@ struct AP {
     const unsigned long long *rec_mask;
     const T *cell;
     const uint8_t *mask;
-    const double *coef;
+    const T *coefT;
     const unsigned long long *excl;
     const DirTables *dt;
     double *grad_eta, *grad_k0;
@@ -50,11 +50,11 @@ struct AP {
 template <typename T, int D, bool CURVED>
 struct ASmem {
     static constexpr int RW = 66;
-    static constexpr int NC = CURVED ? 26 : 1;
     /* adjoint accumulation, slot = row % 3; FP64 in both precisions (LDS
      * FP32 atomic adds measured about 4x slower here than FP64 ones) */
     double A[3][D + 4][RW];
-    double C[NC][64];
+    T C[CURVED ? 25 : 1][64];               /* coefficient series (d_coefT form) */
+    unsigned long long EX[CURVED ? 64 : 1]; /* excised-departure bits */
     double X[RT_FUSED_WAVES][4][64];
     float wrow[10][12] __attribute__((aligned(16)));
 };
@@ -70,7 +70,6 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
 {
     constexpr int DW = D / RT_FUSED_WAVES;
     constexpr int RW = ASmem<T, D, CURVED>::RW;
-    constexpr int NC = ASmem<T, D, CURVED>::NC;
     constexpr int NA = (D + 4) * RW;
     __shared__ ASmem<T, D, CURVED> sm;
 
@@ -98,11 +97,13 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
     }
     auto stage_coef = [&](int row) {
         if constexpr (CURVED) {
-            const int n = (int)(min(64, nx - 1 - (col0 + 1)) * 2);
+            const int ncell = min(64, nx - 2 - col0);
             const size_t off = (size_t)row * nx + col0 + 1;
-            for (int e = wv; e < NC; e += RT_FUSED_WAVES) {
-                const void *src = e < 25 ? (const void *)(p.coef + (size_t)e * N + off) : (const void *)(p.excl + off);
-                lds_dma_dwords(src, &sm.C[e][0], n, lane);
+            for (int e = wv; e < 26; e += RT_FUSED_WAVES) {
+                if (e < 25)
+                    lds_dma_dwords(p.coefT + (size_t)e * N + off, &sm.C[e][0], ncell * (int)(sizeof(T) / 4), lane);
+                else
+                    lds_dma_dwords(p.excl + off, &sm.EX[0], 2 * ncell, lane);
             }
         }
     };
@@ -231,9 +232,15 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
         const int slm = (j + 2) % 3, sl0 = j % 3, slp = (j + 1) % 3;
         if constexpr (CURVED) {
             T cS[5], cX[5], cY[5], cU[5], cV[5];
-            cell_series<T>(sm.C, lane, p.startx + gi * p.dx, p.starty + j * p.dy, 1.0 / p.dx, 1.0 / p.dy, cS, cX, cY,
-                           cU, cV);
-            const uint32_t exw = (uint32_t)(__double_as_longlong(sm.C[25][lane]) >> (wv * DW));
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                cS[k] = sm.C[k][lane];
+                cX[k] = sm.C[5 + k][lane];
+                cY[k] = sm.C[10 + k][lane];
+                cU[k] = sm.C[15 + k][lane];
+                cV[k] = sm.C[20 + k][lane];
+            }
+            const uint32_t exw = (uint32_t)(sm.EX[lane] >> (wv * DW));
             const unsigned long long mbits = active ? (p.rec_mask + (size_t)wv * N)[ij] : 0ull;
             const T angu = (sizeof(T) == 4) ? (T)dtab->ang_uf : (T)dtab->ang_u;
             const int quarter = dtab->quarter, nGrid = dtab->nGrid;
@@ -361,7 +368,7 @@ int launch_adj(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, 
     p.rec_mask = rec.mask;
     p.cell = (const T *)c->d_cell;
     p.mask = c->d_mask;
-    p.coef = c->d_coef;
+    p.coefT = (const T *)c->d_coefT;
     p.excl = c->d_excl;
     p.dt = c->d_dirt;
     p.grad_eta = c->d_grad[0];

@@ -49,6 +49,7 @@ struct KP {
     const T *cell;              /* [RT_CELL_FIELDS][N] */
     const uint8_t *mask;
     const double *coef;         /* [25][N] FP64 */
+    const T *coefT;             /* [25][N] in kernel form (S - 1, relative X, Y) */
     const unsigned long long *excl; /* [N] excised-departure bits */
     double *mom;                /* [6][N] */
     unsigned long long *stats;  /* [2] */
@@ -73,9 +74,10 @@ struct KP {
 template <typename T, int D, bool CURVED>
 struct Smem {
     static constexpr int RW = 66;
-    static constexpr int NC = CURVED ? 26 : 1; /* 25 Fourier coefficients + exclusion bits */
     T I[3][D + 4][RW];
-    double C[NC][64];
+    T C[CURVED ? 25 : 1][64];                    /* coefficient series (d_coefT form) */
+    unsigned long long EX[CURVED ? 64 : 1];      /* excised-departure bits */
+    T F[RT_CELL_FIELDS][64];                     /* per-cell fields */
     double X[RT_FUSED_WAVES][6][64];
     float wrow[10][12] __attribute__((aligned(16)));
 };
@@ -206,6 +208,7 @@ KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, con
     p.cell = (const T *)c->d_cell;
     p.mask = c->d_mask;
     p.coef = c->d_coef;
+    p.coefT = (const T *)c->d_coefT;
     p.excl = c->d_excl;
     p.mom = c->d_mom;
     p.stats = c->d_stats;
@@ -295,6 +298,44 @@ int launch_fused_step_rec(rt_ctx *c, const void *in, void *out, const RecordPtrs
                       : dispatch_d<float, false, true>(c, in, out, false, &rec);
     return curved ? dispatch_d<double, true, true>(c, in, out, false, &rec)
                   : dispatch_d<double, false, true>(c, in, out, false, &rec);
+}
+
+/* The fused kernel's form of the Fourier tables (computed once per table
+ * build): S - 1, X and Y relative to the cell centre in cell units, the
+ * velocity series unchanged, in the kernel precision. */
+template <typename T>
+__global__ void k_coef_rel(const double *__restrict__ coef, T *__restrict__ out, int nx, long N, double startx,
+                           double starty, double dx, double dy)
+{
+    const long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (ij >= N) return;
+    const int gi = (int)(ij % nx), j = (int)(ij / nx);
+    const double xc = startx + gi * dx, yc = starty + j * dy;
+    const double idx = 1.0 / dx, idy = 1.0 / dy;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        out[(size_t)k * N + ij] = (T)(k == 0 ? coef[ij] - 1.0 : coef[(size_t)k * N + ij]);
+        out[(size_t)(5 + k) * N + ij] = (T)((k == 0 ? coef[5 * (size_t)N + ij] - xc : coef[(size_t)(5 + k) * N + ij]) * idx);
+        out[(size_t)(10 + k) * N + ij] =
+            (T)((k == 0 ? coef[10 * (size_t)N + ij] - yc : coef[(size_t)(10 + k) * N + ij]) * idy);
+        out[(size_t)(15 + k) * N + ij] = (T)coef[(size_t)(15 + k) * N + ij];
+        out[(size_t)(20 + k) * N + ij] = (T)coef[(size_t)(20 + k) * N + ij];
+    }
+}
+
+int launch_coef_rel(rt_ctx *c)
+{
+    if (c->desc.streaming != RT_CURVED_FIXED || c->nCoeff != 5) return RT_OK;
+    const rt_desc &d = c->desc;
+    const unsigned nb = (unsigned)((c->N + 255) / 256);
+    if (c->fp32)
+        k_coef_rel<float><<<nb, 256, 0, c->stream>>>(c->d_coef, (float *)c->d_coefT, d.nx, c->N, d.startx, d.starty,
+                                                     d.dx, d.dy);
+    else
+        k_coef_rel<double><<<nb, 256, 0, c->stream>>>(c->d_coef, (double *)c->d_coefT, d.nx, c->N, d.startx, d.starty,
+                                                      d.dx, d.dy);
+    RT_HIP(c, hipGetLastError());
+    return RT_OK;
 }
 
 /* ---- batches of independent problems ------------------------------------ */

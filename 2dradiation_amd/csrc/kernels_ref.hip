@@ -108,7 +108,7 @@ template <typename T>
 __global__ void k_prepare(const double *__restrict__ f, const double *__restrict__ alpha,
                           const double *__restrict__ t11, const double *__restrict__ t12,
                           const double *__restrict__ t21, const double *__restrict__ t22, T *__restrict__ cell,
-                          int N, double dt)
+                          const uint8_t *__restrict__ mask, int N, double dt)
 {
     int ij = blockIdx.x * blockDim.x + threadIdx.x;
     if (ij >= N) return;
@@ -124,6 +124,7 @@ __global__ void k_prepare(const double *__restrict__ f, const double *__restrict
     cell[CF_UY * (size_t)N + ij] = (T)u2;
     cell[CF_ADT * (size_t)N + ij] = (T)(a * dt);
     cell[CF_IA3 * (size_t)N + ij] = (T)(1.0 / (a * a * a));
+    cell[CF_ACT * (size_t)N + ij] = (mask[ij] & MASK_ACTIVE) ? T(1) : T(0);
 }
 
 /* ---- reference-order stream (src/Radiation.cpp:380-414) ---- */
@@ -695,10 +696,10 @@ int launch_prepare_cells(rt_ctx *c)
 {
     if (c->fp32)
         k_prepare<float><<<nblk(c->N, 256), 256, 0, c->stream>>>(c->d_f64, c->d_alpha, c->d_t11, c->d_t12, c->d_t21, c->d_t22,
-                                                                 (float *)c->d_cell, c->N, c->desc.dt);
+                                                                 (float *)c->d_cell, c->d_mask, c->N, c->desc.dt);
     else
         k_prepare<double><<<nblk(c->N, 256), 256, 0, c->stream>>>(c->d_f64, c->d_alpha, c->d_t11, c->d_t12, c->d_t21, c->d_t22,
-                                                                  (double *)c->d_cell, c->N, c->desc.dt);
+                                                                  (double *)c->d_cell, c->d_mask, c->N, c->desc.dt);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }

@@ -38,7 +38,7 @@ struct DirTables {
 
 /* Per-cell inputs of the fused kernel, SoA [RT_CELL_FIELDS][N] in the kernel
  * precision. */
-enum { CF_ETA = 0, CF_K0, CF_K1, CF_KA, CF_UX, CF_UY, CF_ADT, CF_IA3, RT_CELL_FIELDS };
+enum { CF_ETA = 0, CF_K0, CF_K1, CF_KA, CF_UX, CF_UY, CF_ADT, CF_IA3, CF_ACT, RT_CELL_FIELDS };
 /* Cell mask bits. */
 enum { MASK_ACTIVE = 1, MASK_NEAR_BH = 2 };
 
@@ -64,6 +64,9 @@ struct rt_ctx {
     bool has_geometry = false, has_tables = false;
     double *d_alpha = nullptr, *d_t11 = nullptr, *d_t12 = nullptr, *d_t21 = nullptr, *d_t22 = nullptr;
     double *d_coef = nullptr;     /* SoA [5 series][nCoeff][N], FP64 */
+    /* the same coefficients in the fused kernel's form and precision: S - 1,
+     * X and Y relative to the cell centre in cells, velocity as is */
+    void *d_coefT = nullptr;
     uint8_t *d_mask = nullptr;
     double max_disp = 0;          /* max |departure - cell| in cells (curved tables) */
     /* bit d set: the departure point of direction d (< 64) is excised */
@@ -136,6 +139,7 @@ extern "C" int prepare_fused(rt_ctx *c);
 int launch_geodesic_tables(rt_ctx *c, const double *d_metric, double tol);
 int launch_departure_bound(rt_ctx *c, double *max_disp);
 int launch_fill(rt_ctx *c, double *p, size_t n, double v);
+int launch_coef_rel(rt_ctx *c);
 int launch_adjoint_step(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, const RecordPtrs &rec);
 int launch_sample(rt_ctx *c, const void *I, int scaled, int npts, const double *px, const double *py, double *out);
 int launch_sample_adjoint(rt_ctx *c, void *lam, int scaled, int npts, const double *px, const double *py,
