@@ -58,7 +58,7 @@ __device__ inline T angle_index_bf(T vx, T vy, T u, int quarter)
     const int ky = 3 - 2 * py;
     const int k = swap ? ky : kx;
     const T up = u * p;
-    const T dd = T(k * quarter) + (swap ? -up : up);
+    const T dd = T(__mul24(k, quarter)) + (swap ? -up : up);
     return (vx == T(0) && vy == T(0)) ? T(0) : dd;
 }
 
@@ -81,12 +81,14 @@ __device__ inline void curved_geometry(const T (&cS)[5], const T (&cX)[5], const
                                        const T (&cV)[5], CDirTables *dtab, CDirTables *dtab0, const float *wrow,
                                        int d, T angu, int quarter, int nGrid, T ddmax, DirGeom<T> &G)
 {
+    /* Fourier basis 0 is 1, and fma(c0, 1, 0) = c0 exactly, so the series
+     * start from c0 */
     T B[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++) B[q] = (sizeof(T) == 4) ? (T)dtab->basisf[d][q] : (T)dtab->basis[d][q];
-    T s = 0, di = 0, dj = 0, vx = 0, vy = 0;
+    for (int q = 1; q < 5; q++) B[q] = (sizeof(T) == 4) ? (T)dtab->basisf[d][q] : (T)dtab->basis[d][q];
+    T s = cS[0], di = cX[0], dj = cY[0], vx = cU[0], vy = cV[0];
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
+    for (int q = 1; q < 5; q++) {
         s += cS[q] * B[q];
         di += cX[q] * B[q];
         dj += cY[q] * B[q];
@@ -106,7 +108,10 @@ __device__ inline void curved_geometry(const T (&cS)[5], const T (&cX)[5], const
     const T t = dd - T(g0);
     g0 = (unsigned)g0 < (unsigned)nGrid ? g0 : 0; /* dd == nGrid after rounding is angle 0 */
     const int q10 = (int)(__umul24((unsigned)g0, 6554u) >> 16); /* g0 / 10 for g0 < 16384 */
-    const int r = g0 - 10 * q10;
+    /* r = g0 - 10 q10 as a full-rate 24-bit mad (the compiler otherwise
+     * forms a quarter-rate 64-bit mad) */
+    int r;
+    asm("v_mad_i32_i24 %0, %1, -10, %2" : "=v"(r) : "v"(q10), "v"(g0));
     int woff;
     if constexpr (sizeof(T) == 4) {
         const float4 *wr = (const float4 *)((const char *)wrow + __umul24((unsigned)r, 48u));
