@@ -76,25 +76,12 @@ struct DirGeom {
     bool lo_i, lo_j;
 };
 
+/* Geometry from the five series values at the direction's angle: s - 1,
+ * departure offset (di, dj) in cells, departure velocity (vx, vy). */
 template <typename T, int D>
-__device__ inline void curved_geometry(const T (&cS)[5], const T (&cX)[5], const T (&cY)[5], const T (&cU)[5],
-                                       const T (&cV)[5], CDirTables *dtab, CDirTables *dtab0, const float *wrow,
-                                       int d, T angu, int quarter, int nGrid, T ddmax, DirGeom<T> &G)
+__device__ inline void geometry_from_series(T s, T di, T dj, T vx, T vy, CDirTables *dtab0, const float *wrow,
+                                            T angu, int quarter, int nGrid, T ddmax, DirGeom<T> &G)
 {
-    /* Fourier basis 0 is 1, and fma(c0, 1, 0) = c0 exactly, so the series
-     * start from c0 */
-    T B[5];
-#pragma unroll
-    for (int q = 1; q < 5; q++) B[q] = (sizeof(T) == 4) ? (T)dtab->basisf[d][q] : (T)dtab->basis[d][q];
-    T s = cS[0], di = cX[0], dj = cY[0], vx = cU[0], vy = cV[0];
-#pragma unroll
-    for (int q = 1; q < 5; q++) {
-        s += cS[q] * B[q];
-        di += cX[q] * B[q];
-        dj += cY[q] * B[q];
-        vx += cU[q] * B[q];
-        vy += cV[q] * B[q];
-    }
     G.s = s;
     const T fi = fmin(fmax(floor(di), T(-1)), T(0));
     const T fj = fmin(fmax(floor(dj), T(-1)), T(0));
@@ -131,6 +118,46 @@ __device__ inline void curved_geometry(const T (&cS)[5], const T (&cX)[5], const
     }
     int s0 = q10 + woff;
     G.s0 = s0 < 0 ? s0 + D : (s0 >= D ? s0 - D : s0);
+}
+
+template <typename T, int D>
+__device__ inline void curved_geometry(const T (&cS)[5], const T (&cX)[5], const T (&cY)[5], const T (&cU)[5],
+                                       const T (&cV)[5], CDirTables *dtab, CDirTables *dtab0, const float *wrow,
+                                       int d, T angu, int quarter, int nGrid, T ddmax, DirGeom<T> &G)
+{
+    /* Fourier basis 0 is 1, and fma(c0, 1, 0) = c0 exactly, so the series
+     * start from c0 */
+    T B[5];
+#pragma unroll
+    for (int q = 1; q < 5; q++) B[q] = (sizeof(T) == 4) ? (T)dtab->basisf[d][q] : (T)dtab->basis[d][q];
+    T s = cS[0], di = cX[0], dj = cY[0], vx = cU[0], vy = cV[0];
+#pragma unroll
+    for (int q = 1; q < 5; q++) {
+        s += cS[q] * B[q];
+        di += cX[q] * B[q];
+        dj += cY[q] * B[q];
+        vx += cU[q] * B[q];
+        vy += cV[q] * B[q];
+    }
+    geometry_from_series<T, D>(s, di, dj, vx, vy, dtab0, wrow, angu, quarter, nGrid, ddmax, G);
+}
+
+/* The five FP32 Fourier series of this lane's cell at 4 directions with
+ * v_mfma_f32_4x4x1f32 (16 blocks of 4x4, K = 1): lane 4b+j supplies the
+ * coefficient of its own cell as B[0][j] of block b and basis value of
+ * direction i = lane & 3 as A[i][0]; register i of lane 4b+j receives
+ * D[i][j] = C[i][j] + A[i] B[j].  Starting from C = 0 with A = 1 for
+ * coefficient 0, the chain gives every series the bits of the FMA chain
+ * c0, fma(c_q, B_q, .) for q = 1..4.  bq: basis values 1..4 of direction
+ * (lane & 3) of the group. */
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ inline f32x4 series4_mfma(const float (&c)[5], const float (&bq)[4])
+{
+    f32x4 r = {0.f, 0.f, 0.f, 0.f};
+    r = __builtin_amdgcn_mfma_f32_4x4x1f32(1.0f, c[0], r, 0, 0, 0);
+#pragma unroll
+    for (int q = 1; q < 5; q++) r = __builtin_amdgcn_mfma_f32_4x4x1f32(bq[q - 1], c[q], r, 0, 0, 0);
+    return r;
 }
 
 } // namespace rtf
