@@ -144,7 +144,8 @@ int build_dir_tables(rt_ctx *c, const double *w, const double *phi, const double
                 pmin = std::min(pmin, k);
                 pmax = std::max(pmax, k);
             }
-        if (pmax < 0 || pmax - pmin > 4) {
+        /* the fused kernels' LDS ring holds the windows of offsets -2 and -1 */
+        if (pmax < 0 || pmax - pmin > 4 || pmin - 4 < -RT_RING_PAD || pmin - 4 > -1) {
             ok = false;
             break;
         }

@@ -93,7 +93,7 @@ __device__ inline void geometry_from_series(T s, T di, T dj, T vx, T vy, CDirTab
     dd = fmin(fmax(dd, T(1e-8)), ddmax);
     int g0 = (int)dd;
     const T t = dd - T(g0);
-    g0 = (unsigned)g0 < (unsigned)nGrid ? g0 : 0; /* dd == nGrid after rounding is angle 0 */
+    g0 = min(max(g0, 0), nGrid); /* g0 == nGrid (dd rounded up) is angle 0: rows D..D+5 repeat -2..3 */
     const int q10 = (int)(__umul24((unsigned)g0, 6554u) >> 16); /* g0 / 10 for g0 < 16384 */
     /* r = g0 - 10 q10 as a full-rate 24-bit mad (the compiler otherwise
      * forms a quarter-rate 64-bit mad) */
@@ -116,8 +116,10 @@ __device__ inline void geometry_from_series(T s, T di, T dj, T vx, T vy, CDirTab
         for (int m = 0; m < 5; m++) G.Cw[m] = dtab0->wA[r][m] + t * dtab0->wD[r][m];
         woff = dtab0->woff[r];
     }
-    int s0 = q10 + woff;
-    G.s0 = s0 < 0 ? s0 + D : (s0 >= D ? s0 - D : s0);
+    /* LDS row of the window start: woff is in [-2, -1] (checked when the
+     * tables are built) and q10 in [0, D], so the window rows lie in
+     * [0, D+5] without wrapping */
+    G.s0 = q10 + woff + RT_RING_PAD;
 }
 
 template <typename T, int D>

@@ -52,7 +52,7 @@ struct ASmem {
     static constexpr int RW = 66;
     /* adjoint accumulation, slot = row % 3; FP64 in both precisions (LDS
      * FP32 atomic adds measured about 4x slower here than FP64 ones) */
-    double A[3][D + 4][RW];
+    double A[3][RT_RING_ROWS(D)][RW];
     T C[CURVED ? 25 : 1][64];               /* coefficient series (d_coefT form) */
     unsigned long long EX[CURVED ? 64 : 1]; /* excised-departure bits */
     double X[RT_FUSED_WAVES][4][64];
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
 {
     constexpr int DW = D / RT_FUSED_WAVES;
     constexpr int RW = ASmem<T, D, CURVED>::RW;
-    constexpr int NA = (D + 4) * RW;
+    constexpr int NA = RT_RING_ROWS(D) * RW;
     __shared__ ASmem<T, D, CURVED> sm;
 
     const int lane = threadIdx.x & 63;
@@ -116,7 +116,8 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
             const double v = slot[e];
             const int col = col0 + cidx;
             if (v != 0.0 && col < nx) {
-                const int d = r < D ? r : r - D;
+                int d = r - RT_RING_PAD;
+                d = d < 0 ? d + D : (d >= D ? d - D : d);
                 unsafeAtomicAdd(p.lam_out + (size_t)d * N + (size_t)row * nx + col, (T)v);
             }
             slot[e] = 0.0;
@@ -289,10 +290,11 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
                     const T f = (T)jb;
                     const int ra = oj < 0 ? slm : sl0, rb = oj < 0 ? sl0 : slp;
                     const int ca = lane + 1 + oi;
-                    atomicAdd(&sm.A[ra][d][ca], (double)(f * (T(1) - fx) * (T(1) - fy)));
-                    atomicAdd(&sm.A[ra][d][ca + 1], (double)(f * fx * (T(1) - fy)));
-                    atomicAdd(&sm.A[rb][d][ca], (double)(f * (T(1) - fx) * fy));
-                    atomicAdd(&sm.A[rb][d][ca + 1], (double)(f * fx * fy));
+                    const int dr = d + RT_RING_PAD;
+                    atomicAdd(&sm.A[ra][dr][ca], (double)(f * (T(1) - fx) * (T(1) - fy)));
+                    atomicAdd(&sm.A[ra][dr][ca + 1], (double)(f * fx * (T(1) - fy)));
+                    atomicAdd(&sm.A[rb][dr][ca], (double)(f * (T(1) - fx) * fy));
+                    atomicAdd(&sm.A[rb][dr][ca + 1], (double)(f * fx * fy));
                 }
             }
         }

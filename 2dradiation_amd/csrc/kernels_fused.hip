@@ -3,8 +3,8 @@
  *
  * One workgroup = 4 wavefronts walks a 64-cell-wide column strip of the grid
  * down a chunk of rows.  Three input rows (all directions) live in an LDS ring
- * [slot][direction][66] (64 cells plus one halo cell each side, directions
- * 0..3 repeated after D-1); the row two ahead is loaded straight from HBM
+ * [slot][direction][66] (64 cells plus one halo cell each side, direction
+ * rows -2..D+3 taken mod D); the row two ahead is loaded straight from HBM
  * into the slot that just died (global_load_lds, no register staging) while
  * the current row is computed.  Lane = cell column, wavefront w = direction
  * block [w*D/4, (w+1)*D/4).
@@ -12,7 +12,8 @@
  * Stream (reference src/Radiation.cpp:380-414 flat, :454-512 curved):
  *   flat:   bilinear of the same direction at the departure point x - c dt.
  *   curved: the five geodesic Fourier series (S, X, Y, Cx, Cy) are evaluated
- *           at phi_d with the per-direction basis from scalar registers; the
+ *           at phi_d (FP32: four directions at a time on the matrix cores,
+ *           FP64: FMA chains with the basis in scalar registers); the
  *           departure velocity angle (Hastings atan2) picks the cubic angular
  *           window; the lerp between the two cubic stencils is folded into 5
  *           combined taps; each of the 4 bilinear corners is clamped >= 0.
@@ -67,14 +68,14 @@ struct KP {
 };
 
 /* LDS of one workgroup.  Intensity rows are SoA [direction][66 columns]
- * with 4 wrap-around direction rows (D..D+3 repeat 0..3), so the 5 angular
- * taps of a corner are 5 rows of one column at constant offsets; lanes
+ * holding directions -2..D+3 (mod D), so the 5 angular taps of a corner are
+ * 5 rows of one column at constant offsets, never wrapped; lanes
  * (= columns) hit consecutive banks.  Rows and FP64 coefficients arrive by
  * direct global-to-LDS loads, without passing through registers. */
 template <typename T, int D, bool CURVED>
 struct Smem {
     static constexpr int RW = 66;
-    T I[3][D + 4][RW];
+    T I[3][RT_RING_ROWS(D)][RW];
     T C[CURVED ? 25 : 1][64];                    /* coefficient series (d_coefT form) */
     unsigned long long EX[CURVED ? 64 : 1];      /* excised-departure bits */
     T F[RT_CELL_FIELDS][64];                     /* per-cell fields */

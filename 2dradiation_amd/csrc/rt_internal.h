@@ -12,6 +12,12 @@
 #define RT_MAX_FUSED_DIR 64
 #define RT_FUSED_STRIP 64 /* cells per wavefront row segment */
 #define RT_FUSED_WAVES 4
+/* LDS intensity rows of one grid row: directions -2..D+3 (mod D), so that
+ * every 5-tap angular window [s0, s0+4] with s0 in [-2, D-1] (and the window
+ * of fine angle nGrid, which equals angle 0) is contiguous; the window of
+ * direction s0 starts at LDS row s0 + RT_RING_PAD */
+#define RT_RING_PAD 2
+#define RT_RING_ROWS(D) ((D) + 6)
 
 /* Per-direction constants of the fused kernel, uploaded once per stencil.
  * Indexing by a wave-uniform direction lets the compiler use scalar loads. */
