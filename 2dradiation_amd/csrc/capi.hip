@@ -801,19 +801,25 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
         r.mask = c->d_rec_mask + (size_t)i * RT_FUSED_WAVES * N;
         return r;
     };
+    /* halo and excised cells keep their values in every state: copy them
+     * into the segment buffers once */
+    for (int i = 0; i < std::min(every, c->ck_steps); i++)
+        TRY(launch_copy_fixed_cells(c, c->d_ckpt, (char *)c->d_seg + (size_t)i * bytes));
+    /* the adjoint step clears the active cells of the buffer it reads, which
+     * then takes the next step's scatter; what accumulates in fixed cells is
+     * never read (fixed cells do not depend on active ones) */
+    RT_HIP(c, hipMemsetAsync(c->d_lam[1 - lc], 0, bytes, c->stream));
     for (int s = c->ck_count - 1; s >= 0; s--) {
         const int steps = std::min(every, c->ck_steps - s * every);
         /* recompute the segment, keeping every state after a step */
         const void *prev = (const char *)c->d_ckpt + (size_t)s * bytes;
         for (int i = 0; i < steps; i++) {
             void *out = (char *)c->d_seg + (size_t)i * bytes;
-            TRY(launch_copy_fixed_cells(c, prev, out));
             TRY(launch_fused_step_rec(c, prev, out, rec_at(i)));
             prev = out;
         }
         /* adjoint steps, last to first */
         for (int i = steps - 1; i >= 0; i--) {
-            RT_HIP(c, hipMemsetAsync(c->d_lam[1 - lc], 0, bytes, c->stream));
             TRY(launch_adjoint_step(c, c->d_lam[lc], c->d_lam[1 - lc], (const char *)c->d_seg + (size_t)i * bytes,
                                     rec_at(i)));
             lc = 1 - lc;

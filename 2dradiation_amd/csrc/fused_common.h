@@ -153,6 +153,22 @@ __device__ inline void curved_geometry(const T (&cS)[5], const T (&cX)[5], const
  * c0, fma(c_q, B_q, .) for q = 1..4.  bq: basis values 1..4 of direction
  * (lane & 3) of the group. */
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+/* The A operands of series4_mfma for the NG direction groups of wavefront
+ * wv (DW directions): basis values 1..4 of direction 4g + (lane & 3), 0 past
+ * the wavefront's last direction. */
+template <int DW, int NG>
+__device__ inline void basis_groups(CDirTables *dtab, int wv, int lane, float (&bas4)[NG][4])
+{
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int i = 4 * g + (lane & 3);
+        const int d = wv * DW + (i < DW ? i : 0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) bas4[g][q] = i < DW ? dtab->basisf[d][q + 1] : 0.f;
+    }
+}
+
 __device__ inline f32x4 series4_mfma(const float (&c)[5], const float (&bq)[4])
 {
     f32x4 r = {0.f, 0.f, 0.f, 0.f};

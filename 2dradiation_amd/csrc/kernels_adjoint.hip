@@ -29,7 +29,9 @@ using namespace rtf;
 
 template <typename T>
 struct AP {
-    const T *lam_in;   /* [D][N] adjoint of the state after the step (state units) */
+    T *lam_in;         /* [D][N] adjoint of the state after the step (state units); its
+                        * active cells are cleared once read, so that the buffer can
+                        * take the next step's scatter */
     T *lam_out;        /* [D][N] adjoint of the state before the step (accumulated) */
     const T *Inext;    /* [D][N] state after the step */
     const double *rec_m;
@@ -92,6 +94,10 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
     if (rowA >= rowB) return;
 
     for (int e = threadIdx.x; e < 3 * NA; e += 256) (&sm.A[0][0][0])[e] = 0.0;
+    /* FP32 curved: A operands of the matrix-core series (as in the forward) */
+    constexpr int NG = (DW + 3) / 4;
+    float bas4[(CURVED && sizeof(T) == 4) ? NG : 1][4];
+    if constexpr (CURVED && sizeof(T) == 4) basis_groups<DW, NG>(dtab0, wv, lane, bas4);
     if constexpr (CURVED && sizeof(T) == 4) {
         if (threadIdx.x < 120) (&sm.wrow[0][0])[threadIdx.x] = (&dtab0->wrowf[0][0])[threadIdx.x];
     }
@@ -99,11 +105,19 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
         if constexpr (CURVED) {
             const int ncell = min(64, nx - 2 - col0);
             const size_t off = (size_t)row * nx + col0 + 1;
+            constexpr int E = (int)(sizeof(T) / 4);
+#pragma unroll 1
             for (int e = wv; e < 26; e += RT_FUSED_WAVES) {
-                if (e < 25)
-                    lds_dma_dwords(p.coefT + (size_t)e * N + off, &sm.C[e][0], ncell * (int)(sizeof(T) / 4), lane);
-                else
-                    lds_dma_dwords(p.excl + off, &sm.EX[0], 2 * ncell, lane);
+                const uint32_t *src = e < 25 ? (const uint32_t *)(p.coefT + (size_t)e * N + off)
+                                             : (const uint32_t *)(p.excl + off);
+                uint32_t *dst = e < 25 ? (uint32_t *)&sm.C[e][0] : (uint32_t *)&sm.EX[0];
+                const int n = (e < 25 ? E : 2) * ncell;
+                if (ncell == 64) {
+                    /* full strip: whole unmasked instructions */
+                    for (int b = 0; b < n; b += 64) __builtin_amdgcn_global_load_lds(src + b + lane, dst + b, 4, 0, 0);
+                } else {
+                    lds_dma_dwords(src, dst, n, lane);
+                }
             }
         }
     };
@@ -142,6 +156,7 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
         for (int k = 0; k < DW; k++) {
             const int d = wv * DW + k;
             lam[k] = active ? (p.lam_in + (size_t)d * N)[ij] : T(0);
+            if (active) (p.lam_in + (size_t)d * N)[ij] = T(0);
             const T In = active ? (p.Inext + (size_t)d * N)[ij] : T(0);
             const double l = (double)lam[k];
             P0 += l;
@@ -246,16 +261,32 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
             const T angu = (sizeof(T) == 4) ? (T)dtab->ang_uf : (T)dtab->ang_u;
             const int quarter = dtab->quarter, nGrid = dtab->nGrid;
             const T ddmax = T(nGrid) - T(1e-8);
+            f32x4 ser[5];
 #pragma unroll
             for (int k = 0; k < DW; k++) {
                 const int d = wv * DW + k;
+                if constexpr (sizeof(T) == 4) {
+                    if (k % 4 == 0) {
+                        ser[0] = series4_mfma(cS, bas4[k / 4]);
+                        ser[1] = series4_mfma(cX, bas4[k / 4]);
+                        ser[2] = series4_mfma(cY, bas4[k / 4]);
+                        ser[3] = series4_mfma(cU, bas4[k / 4]);
+                        ser[4] = series4_mfma(cV, bas4[k / 4]);
+                    }
+                }
                 const double jb = (double)lam[k] + a3 * dtab->w6[d][0] *
                                                        (gb[0] + gb[1] * dtab->cx[d] + gb[2] * dtab->cy[d]);
                 const bool go = active && !((exw >> k) & 1u) && jb != 0.0;
                 if (__any(go)) {
                     DirGeom<T> G;
-                    curved_geometry<T, D>(cS, cX, cY, cU, cV, dtab, dtab0, &sm.wrow[0][0], d, angu, quarter, nGrid,
-                                          ddmax, G);
+                    if constexpr (sizeof(T) == 4) {
+                        const int i = k % 4;
+                        geometry_from_series<T, D>(ser[0][i], ser[1][i], ser[2][i], ser[3][i], ser[4][i], dtab0,
+                                                   &sm.wrow[0][0], angu, quarter, nGrid, ddmax, G);
+                    } else {
+                        curved_geometry<T, D>(cS, cX, cY, cU, cV, dtab, dtab0, &sm.wrow[0][0], d, angu, quarter,
+                                              nGrid, ddmax, G);
+                    }
                     if (go) {
                         const T s3 = T(1) + G.s * (T(3) + G.s * (T(3) + G.s));
                         const T f = (T)jb * s3;
@@ -362,7 +393,7 @@ int launch_adj(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, 
 {
     const rt_desc &d = c->desc;
     AP<T> p;
-    p.lam_in = (const T *)lam_in;
+    p.lam_in = (T *)const_cast<void *>(lam_in);
     p.lam_out = (T *)lam_out;
     p.Inext = (const T *)Inext;
     p.rec_m = rec.m;
