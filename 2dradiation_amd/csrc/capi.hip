@@ -237,6 +237,8 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
     c->D = d.nDir;
     c->nCoeff = d.nCoeff;
     c->fp32 = d.precision == RT_FP32_MIXED;
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, d.device) != hipSuccess || c->ncu < 1)
+        c->ncu = 256;
     int r = RT_OK;
     auto fail = [&](int rc) {
         g_create_error = c->err;

@@ -411,7 +411,16 @@ int launch_adj(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, 
     p.N = (long)d.nx * d.ny;
     const int strips = (d.nx - 2 + 63) / 64;
     const int rows = d.ny - 2;
-    int chunks = (1024 + strips - 1) / strips;
+    /* one round of resident workgroups (as the forward step) */
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjoint<T, D, CURVED>, 256, 0) != hipSuccess || n < 1)
+            n = 1;
+        per_cu = n;
+    }
+    const int wgs = per_cu * c->ncu;
+    int chunks = (wgs + strips - 1) / strips;
     int H = (rows + chunks - 1) / chunks;
     H = H < 4 ? 4 : (H > 64 ? 64 : H);
     chunks = (rows + H - 1) / H;

@@ -198,9 +198,10 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
 }
 
 /* Kernel parameters of one fused step of context c; H = rows per workgroup
- * (0: about 1024 workgroups with at least 4 rows each). */
+ * (0: about `wgs` workgroups with 4 to 64 rows each). */
 template <typename T>
-KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, const RecordPtrs *rec, int H = 0)
+KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, const RecordPtrs *rec, int H = 0,
+              int wgs = 1024)
 {
     const rt_desc &d = c->desc;
     KP<T> p;
@@ -220,7 +221,7 @@ KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, con
     p.strips = (d.nx - 2 + 63) / 64;
     const int rows = d.ny - 2;
     if (H <= 0) {
-        const int target = (1024 + p.strips - 1) / p.strips;
+        const int target = (wgs + p.strips - 1) / p.strips;
         H = (rows + target - 1) / target;
         H = H < 4 ? 4 : (H > 64 ? 64 : H);
     }
@@ -244,7 +245,16 @@ KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, con
 template <typename T, int D, bool CURVED, bool REC>
 int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments, const RecordPtrs *rec)
 {
-    KP<T> p = make_kp<T>(c, in, out, want_moments, rec);
+    /* one round of resident workgroups: each walks a long chunk of rows,
+     * which amortises the two halo rows it stages first */
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<T, D, CURVED, REC>, 256, 0) != hipSuccess || n < 1)
+            n = 1;
+        per_cu = n;
+    }
+    KP<T> p = make_kp<T>(c, in, out, want_moments, rec, 0, per_cu * c->ncu);
     k_fused<T, D, CURVED, REC><<<p.nblocks, 256, 0, c->stream>>>(p);
     RT_HIP(c, hipGetLastError());
     return RT_OK;

@@ -51,6 +51,7 @@ enum { MASK_ACTIVE = 1, MASK_NEAR_BH = 2 };
 struct rt_ctx {
     rt_desc desc;
     int N = 0, D = 0, nCoeff = 0;
+    int ncu = 0; /* compute units of the device */
     bool fp32 = false;
     hipStream_t stream = nullptr;
     std::string err;
