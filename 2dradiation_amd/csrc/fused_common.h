@@ -20,6 +20,17 @@ __device__ inline float fast_div<float>(float a, float b)
     return a * __builtin_amdgcn_rcpf(b);
 }
 
+/* Bilinear interpolation as three lerps (6 operations; the reference's
+ * order, rtm::bilinear, takes 8).  With f >= 0 and x, y in [0, 1] every lerp
+ * stays >= 0 after rounding. */
+template <typename T>
+__device__ inline T bilinear_lerp(T x, T y, T f00, T f01, T f10, T f11)
+{
+    const T f0 = fma(x, f10 - f00, f00);
+    const T f1 = fma(x, f11 - f01, f01);
+    return fma(y, f1 - f0, f0);
+}
+
 template <typename T>
 __device__ inline T tmax0(T v)
 {
@@ -43,23 +54,23 @@ __device__ inline void lds_dma_dwords(const void *src, void *dst, int n, int lan
 
 /* Fine angular index nGrid*phi/(2 pi) of Tensor2::Phi (src/DataTypes.hh:166-170)
  * with the Hastings atan2 (src/Utility.hh:220-254), branch-free.  The octant
- * offsets are exact multiples of nGrid/4, so only the polynomial term rounds. */
+ * offsets are exact multiples of nGrid/4, so only the polynomial term rounds.
+ * A zero vector (angle 0) gives NaN here (0/0), which the caller's clamp to
+ * [1e-8, nGrid - 1e-8] turns into 1e-8, the index of angle 0 after that
+ * clamp (IEEE max/min return the non-NaN operand).  q1..q4: k * nGrid/4. */
 template <typename T>
-__device__ inline T angle_index_bf(T vx, T vy, T u, int quarter)
+__device__ inline T angle_index_bf(T vx, T vy, T u, T q1, T q2, T q3, T q4)
 {
     const bool swap = !(fabsf((float)vx) > fabsf((float)vy));
     const T num = swap ? vx : vy, den = swap ? vy : vx;
     const T p = rtm::hastings_atan(fast_div(num, den));
-    /* octant offset in quarters: |vx| > |vy|: 0 (or 4 when p < 0) for vx > 0,
-     * 2 for vx < 0; otherwise 1 for vy > 0, 3 for vy < 0 (and the
-     * polynomial enters with a minus sign) */
-    const int px = (int)(vx > T(0)), pneg = (int)(p < T(0)), py = (int)(vy > T(0));
-    const int kx = 2 - 2 * px + 4 * (px & pneg);
-    const int ky = 3 - 2 * py;
-    const int k = swap ? ky : kx;
+    /* octant offset: |vx| > |vy|: 0 (or 4 quarters when p < 0) for vx > 0,
+     * 2 quarters for vx < 0; otherwise 1 quarter for vy > 0, 3 for vy < 0
+     * (and the polynomial enters with a minus sign) */
+    const T kx = vx > T(0) ? (p < T(0) ? q4 : T(0)) : q2;
+    const T ky = vy > T(0) ? q1 : q3;
     const T up = u * p;
-    const T dd = T(__mul24(k, quarter)) + (swap ? -up : up);
-    return (vx == T(0) && vy == T(0)) ? T(0) : dd;
+    return (swap ? ky : kx) + (swap ? -up : up);
 }
 
 /* Geometry of one (cell, direction) of the curved gather: the ray factor as
@@ -89,7 +100,8 @@ __device__ inline void geometry_from_series(T s, T di, T dj, T vx, T vy, CDirTab
     G.fy = dj - fj;
     G.lo_i = fi < T(0);
     G.lo_j = fj < T(0);
-    T dd = angle_index_bf<T>(vx, vy, angu, quarter);
+    const T q1 = T(quarter);
+    T dd = angle_index_bf<T>(vx, vy, angu, q1, T(2) * q1, T(3) * q1, T(4) * q1);
     dd = fmin(fmax(dd, T(1e-8)), ddmax);
     int g0 = (int)dd;
     const T t = dd - T(g0);
