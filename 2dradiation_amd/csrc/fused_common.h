@@ -104,7 +104,8 @@ __device__ inline void geometry_from_series(T s, T di, T dj, T vx, T vy, CDirTab
     T dd = angle_index_bf<T>(vx, vy, angu, q1, T(2) * q1, T(3) * q1, T(4) * q1);
     dd = fmin(fmax(dd, T(1e-8)), ddmax);
     int g0 = (int)dd;
-    const T t = dd - T(g0);
+    /* dd > 0: dd - floor(dd) is exact, and v_fract_f32 computes it in one step */
+    const T t = (sizeof(T) == 4) ? (T)__builtin_amdgcn_fractf((float)dd) : dd - T(g0);
     g0 = min(max(g0, 0), nGrid); /* g0 == nGrid (dd rounded up) is angle 0: rows D..D+5 repeat -2..3 */
     const int q10 = (int)(__umul24((unsigned)g0, 6554u) >> 16); /* g0 / 10 for g0 < 16384 */
     /* r = g0 - 10 q10 as a full-rate 24-bit mad (the compiler otherwise
