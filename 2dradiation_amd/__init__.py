@@ -74,6 +74,8 @@ def lib():
     L.rt_upload_I.argtypes = [vp, _dp]
     L.rt_download_I.argtypes = [vp, _dp]
     L.rt_get_moments.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.rt_set_lab_frame.argtypes = [vp, C.POINTER(C.POINTER(C.c_double)), _dp, _dp, _dp]
+    L.rt_get_moments_lf.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.rt_iteration_stats.argtypes = [vp, C.POINTER(i32), C.POINTER(d)]
     L.rt_reset_iteration_stats.argtypes = [vp]
     L.rt_step.argtypes = [vp, i32, i32]
@@ -218,6 +220,23 @@ class Context:
     def moments(self):
         arrs = [np.zeros(self.grid.N) for _ in range(6)]
         self._c(lib().rt_get_moments(self.h, *arrs))
+        return np.stack(arrs)
+
+    def set_lab_frame(self, tetrad, gamma):
+        """Full tetrad e[a][A] as [9][N] (00, 01, ..., 22) and gamma_ll as
+        [3][N] (11, 12, 22) for moments_lf (Metric::tetradAB_ul,
+        gammaIJ_ll)."""
+        t = [np.ascontiguousarray(tetrad[k], dtype=np.float64) for k in range(9)]
+        g = [np.ascontiguousarray(gamma[k], dtype=np.float64) for k in range(3)]
+        self._frame = (t, g)  # the library copies; kept for the caller's inspection only
+        arr = (C.POINTER(C.c_double) * 9)(*[x.ctypes.data_as(C.POINTER(C.c_double)) for x in t])
+        self._c(lib().rt_set_lab_frame(self.h, arr, *g))
+
+    def moments_lf(self):
+        """Lab-frame E, Fx, Fy, Pxx, Pxy, Pyy, F of the moments last computed
+        (Radiation::ComputeMomentsLF, src/Radiation.cpp:270-301), [7][N]."""
+        arrs = [np.zeros(self.grid.N) for _ in range(7)]
+        self._c(lib().rt_get_moments_lf(self.h, *arrs))
         return np.stack(arrs)
 
     def iteration_stats(self):

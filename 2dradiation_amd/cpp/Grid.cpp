@@ -1,4 +1,5 @@
 #include "Grid.h"
+#include <fstream>
 
 // Grid::Grid (reference src/Grid.cpp:4-24): nx_ interior points on
 // [start,end] plus a halo cell on each side.
@@ -22,4 +23,31 @@ void Grid::SetCFL(double cfl)
 {
     m_cfl = cfl;
     dt = m_cfl * std::min(dx, dy);
+}
+
+// Grid::WriteFrametoCsv (reference src/Grid.cpp:109-136): one frame as text,
+// file "<directory><name><Format(time, 3)>t <nx>x <ny>y.csv" with a '#'
+// header (grid extent and time) and one line "x,y,r,g,b,a" per cell in
+// storage order, numbers in the stream's default format.
+void Grid::WriteFrametoCsv(float time, const DoubleBuffer &r, const DoubleBuffer &g, const DoubleBuffer &b,
+                           const DoubleBuffer &a, std::string directory, std::string name)
+{
+    CreateDirectory(directory);
+    if (name.empty()) name = "data";
+    std::ofstream out(directory + name + Format(time, 3) + "t " + std::to_string(nx) + "x " + std::to_string(ny) +
+                      "y" + ".csv");
+    out << "#nx=" << nx << "\n";
+    out << "#ny=" << ny << "\n";
+    out << "#startx=" << startx << "\n";
+    out << "#starty=" << starty << "\n";
+    out << "#endx=" << endx << "\n";
+    out << "#endy=" << endy << "\n";
+    out << "#t=" << time << "\n";
+    out << "#x,y,r,g,b,a\n";
+    for (size_t j = 0; j < ny; j++)
+        for (size_t i = 0; i < nx; i++) {
+            const size_t ij = Index(i, j);
+            out << x(i) << "," << y(j) << ",";
+            out << r[ij] << "," << g[ij] << "," << b[ij] << "," << a[ij] << "\n";
+        }
 }

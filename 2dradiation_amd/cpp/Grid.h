@@ -32,6 +32,9 @@ class Grid {
     size_t i(size_t ij) const { return ij % nx; }
     size_t j(size_t ij) const { return ij / nx; }
     Coord ij(const Coord &xy) const { return Coord(i(xy[1]), j(xy[2])); }
+    // Frame output (reference src/Grid.cpp:109-136).
+    void WriteFrametoCsv(float time, const DoubleBuffer &r, const DoubleBuffer &g, const DoubleBuffer &b,
+                         const DoubleBuffer &a, std::string directory, std::string name = "");
     bool OutsideDomain(const Coord &xy) const
     {
         return xy[1] > endx || xy[1] < startx || xy[2] > endy || xy[2] < starty;

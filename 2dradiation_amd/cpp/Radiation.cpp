@@ -4,7 +4,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <ctime>
+#include <fstream>
+#include <iomanip>
 #include <iostream>
+#include <sstream>
 #include "rt2d.h"
 
 void Radiation::Check(int status) const
@@ -90,6 +94,11 @@ Radiation::Radiation(Metric &metric_, Stencil &stencil_, Stencil &streamingStenc
                                    streamingStencil.CyData()));
     Check(rt_set_geometry(ctx, metric.alpha.data(), metric.tetrad11_ul.data(), metric.tetrad12_ul.data(),
                           metric.tetrad21_ul.data(), metric.tetrad22_ul.data()));
+    const double *tetrad[9] = {metric.tetrad00_ul.data(), metric.tetrad01_ul.data(), metric.tetrad02_ul.data(),
+                               metric.tetrad10_ul.data(), metric.tetrad11_ul.data(), metric.tetrad12_ul.data(),
+                               metric.tetrad20_ul.data(), metric.tetrad21_ul.data(), metric.tetrad22_ul.data()};
+    Check(rt_set_lab_frame(ctx, tetrad, metric.gamma11_ll.data(), metric.gamma12_ll.data(),
+                           metric.gamma22_ll.data()));
 }
 
 Radiation::~Radiation()
@@ -147,7 +156,21 @@ void Radiation::ComputeMomentsIF()
     DownloadMoments();
 }
 
-void Radiation::ComputeMomentsLF() { ExitOnError("lab-frame moments are not part of this library"); }
+// Reference: src/Radiation.cpp:270-301.  The lab-frame transform runs on the
+// GPU on the inertial-frame moments of the current I, which also end in
+// E..Pyy.
+void Radiation::ComputeMomentsLF()
+{
+    UploadI();
+    Check(rt_compute_moments(ctx));
+    DownloadMoments();
+    DownloadMomentsLF();
+}
+void Radiation::DownloadMomentsLF()
+{
+    Check(rt_get_moments_lf(ctx, E_LF.data(), Fx_LF.data(), Fy_LF.data(), Pxx_LF.data(), Pxy_LF.data(),
+                            Pyy_LF.data(), F_LF.data()));
+}
 
 // Reference: src/Radiation.cpp:380-414; the streamed intensities end in I
 // and the previous state in Inew, as after the reference's swap.
@@ -201,10 +224,51 @@ void Radiation::Collide()
     averageItterationCount += sum;
 }
 
-// Reference: src/Radiation.cpp:738-885.  The whole step loop runs on the GPU
-// (rt_step: stream, moments and collide fused per step).  Frame output is
-// not part of this library: with writeData set, the final IF moments are
-// left in E..Pyy instead.
+// Log.txt of the reference's Logger::Write (src/Logger.hh:48-97), with the
+// quantities this library has: the stencil's interpolation-error and
+// sigma-table lines and the per-function profiler times are not produced.
+void Radiation::WriteLog(const std::string &directoryPath, const std::string &date, int timeSteps,
+                         double computeSeconds) const
+{
+    CreateDirectory(directoryPath);
+    std::ofstream file(directoryPath + "/Log.txt");
+    file << "Creation Date: " << date << std::endl << std::endl;
+    file << "Spacetime: " << metric.Name() << std::endl;
+    file << "Black Hole  Mass,   M = " << metric.m << std::endl;
+    file << "Black Hole  Spin,   a = " << metric.a << std::endl << std::endl;
+    file << "Grid Structure:" << std::endl;
+    file << "Simulation Time = " << config.simTime << std::endl;
+    file << "(startx,starty) = (" << grid.startx << "," << grid.starty << ")" << std::endl;
+    file << "(endx  ,  endy) = (" << grid.endx << "," << grid.endy << ")" << std::endl;
+    file << "(Nx    ,    Ny) = (" << grid.nx << "," << grid.ny << ")" << std::endl;
+    file << "Nt         = " << timeSteps << std::endl;
+    file << "dx         = " << grid.dx << std::endl;
+    file << "dy         = " << grid.dy << std::endl;
+    file << "dt         = " << grid.dt << std::endl;
+    file << "cfl        = " << grid.GetCFL() << std::endl << std::endl;
+    file << "Stencil Properties:" << std::endl;
+    file << "Intensity Stencil: nDir          = " << stencil.nDir << std::endl;
+    file << "Intensity Stencil: nOrder        = " << stencil.nOrder << std::endl;
+    file << "Intensity Stencil: nCoefficients = " << stencil.nCoefficients << std::endl;
+    file << "Streaming Stencil: nDir          = " << streamingStencil.nDir << std::endl;
+    file << "Streaming Stencil: nOrder        = " << streamingStencil.nOrder << std::endl;
+    file << "Streaming Stencil: nCoefficients = " << streamingStencil.nCoefficients << std::endl << std::endl;
+    file << "Lambda Itteration:" << std::endl;
+    file << "Max Lambda Itteration Count     = " << maxItterationCount << std::endl;
+    file << "Average Lambda Itteration Count = " << averageItterationCount << std::endl << std::endl;
+    file << "Time measurements:" << std::endl;
+    file << "Computation Time: " << computeSeconds << "s" << std::endl;
+    file << "Benchmark: " << grid.nxy * timeSteps / (1e6 * computeSeconds) << " MLUPS" << std::endl;
+}
+
+// Reference: src/Radiation.cpp:738-885.  The steps between two frames run on
+// the GPU as one rt_step call (stream, moments and collide fused per step).
+// A frame is written after the step at which the time since the last frame
+// reaches writePeriod - 1e-8, as in the reference, plus the initial frame
+// (saveInitialData) and a final one; each holds the lab-frame E, Fx, Fy and
+// |F| (ComputeMomentsIF, ComputeMomentsLF, Grid::WriteFrametoCsv) under
+// OUTPUTDIR/<name>/Moments/.  The reported computation time excludes frame
+// output, like the reference's.
 void Radiation::RunSimulation()
 {
     LoadInitialData();
@@ -212,55 +276,88 @@ void Radiation::RunSimulation()
 
     const int timeSteps = (int)std::ceil(config.simTime / grid.dt);
     config.simTime = timeSteps * grid.dt;
+    const std::string directoryPath = OUTPUTDIR + config.name;
+    std::string date;
+    {
+        const std::time_t t = std::time(nullptr);
+        std::ostringstream oss;
+        oss << std::put_time(std::localtime(&t), "%H.%M.%S - %d.%m.%Y");
+        date = oss.str();
+    }
     if (config.printSetup) {
         std::cout << " nx           = " << grid.nx << "\n";
         std::cout << " ny           = " << grid.ny << "\n";
         std::cout << " nDir         = " << stencil.nDir << "\n";
         std::cout << " nFourier     = " << streamingStencil.nDir << "\n";
         std::cout << " simTime      = " << config.simTime << "\n";
+        std::cout << " writePeriod  = " << config.writePeriod << "\n";
         std::cout << " dx           = " << grid.dx << "\n";
         std::cout << " dy           = " << grid.dy << "\n";
         std::cout << " dt           = " << grid.dt << "\n";
-        std::cout << " timeSteps    = " << timeSteps << std::endl;
+        std::cout << " timeSteps    = " << timeSteps << "\n";
+        std::cout << " filesToWrite = " << std::floor(config.simTime / config.writePeriod) << std::endl;
     }
-    if (config.writeData)
-        std::cerr << "note: CSV frame output is not part of this library; final inertial-frame moments are in "
-                     "E, Fx, Fy, Pxx, Pxy, Pyy"
-                  << std::endl;
 
     UploadFields();
     UploadI();
     Check(rt_reset_iteration_stats(ctx));
-    const auto t0 = std::chrono::steady_clock::now();
-    if (!config.updateFourierHarmonics && !config.keepSourceNodesActive) {
-        Check(rt_step(ctx, timeSteps, 1));
-    } else {
-        // per-step hooks of the reference loop (src/Radiation.cpp:790-793)
-        for (int n = 0; n < timeSteps; n++) {
-            if (config.updateFourierHarmonics) UpdateFourierCoefficients();
-            if (config.keepSourceNodesActive) {
-                DownloadI();
-                LoadInitialData();
-                UploadI();
-            }
-            Check(rt_step(ctx, 1, n == timeSteps - 1 ? 1 : 0));
+    double currentTime = config.t0, timeSinceLastFrame = 0, computeSeconds = 0;
+    auto frame = [&]() {
+        Check(rt_compute_moments(ctx));
+        DownloadMoments();
+        DownloadMomentsLF();
+        grid.WriteFrametoCsv((float)currentTime, E_LF, Fx_LF, Fy_LF, F_LF, directoryPath + "/Moments/");
+        timeSinceLastFrame = 0;
+    };
+    if (config.writeData && config.saveInitialData) frame();
+    const bool perStep = config.updateFourierHarmonics || config.keepSourceNodesActive;
+    for (int n = 0; n < timeSteps;) {
+        /* the steps up to the next frame, or to the end */
+        int k = 0;
+        bool writeNow = false;
+        while (n + k < timeSteps && !writeNow) {
+            if (config.printProgress) std::cout << "\n" << n + k << "," << Format(currentTime, 4) << "," << std::flush;
+            k++;
+            currentTime += grid.dt;
+            timeSinceLastFrame += grid.dt;
+            writeNow = config.writeData && timeSinceLastFrame >= config.writePeriod - 1e-8;
         }
+        const int last = (n + k == timeSteps) ? 1 : 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!perStep) {
+            Check(rt_step(ctx, k, last));
+        } else {
+            // per-step hooks of the reference loop (src/Radiation.cpp:790-793)
+            for (int s = 0; s < k; s++) {
+                if (config.updateFourierHarmonics) UpdateFourierCoefficients();
+                if (config.keepSourceNodesActive) {
+                    DownloadI();
+                    LoadInitialData();
+                    UploadI();
+                }
+                Check(rt_step(ctx, 1, last && s == k - 1 ? 1 : 0));
+            }
+        }
+        Check(rt_synchronize(ctx));
+        computeSeconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        n += k;
+        if (writeNow) frame();
     }
-    Check(rt_synchronize(ctx));
-    const auto t1 = std::chrono::steady_clock::now();
-    const double seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (timeSteps > 0 && !(config.writeData && timeSinceLastFrame == 0)) DownloadMoments();
+    if (config.writeData && timeSinceLastFrame != 0) frame();
     DownloadI();
-    if (timeSteps > 0) DownloadMoments();
 
     int32_t mx = 0;
     double sum = 0;
     Check(rt_iteration_stats(ctx, &mx, &sum));
     maxItterationCount = mx;
     averageItterationCount = timeSteps > 0 ? sum / ((double)timeSteps * (grid.nx - 2) * (grid.ny - 2)) : 0.0;
+    std::cout << "total time: " << computeSeconds << "s" << std::endl;
     if (config.printResults) {
-        std::cout << "total time: " << seconds << "s" << std::endl;
         std::cout << "Max Lambda Itteration Count     = " << maxItterationCount << std::endl;
         std::cout << "Average Lambda Itteration Count = " << averageItterationCount << std::endl;
-        std::cout << "Benchmark: " << grid.nxy * timeSteps / (1e6 * seconds) << "MLUPS" << std::endl;
+        std::cout << "Computation Time: " << computeSeconds << "s" << std::endl;
+        std::cout << "Benchmark: " << grid.nxy * timeSteps / (1e6 * computeSeconds) << "MLUPS" << std::endl;
     }
+    if (config.writeData) WriteLog(directoryPath, date, timeSteps, computeSeconds);
 }

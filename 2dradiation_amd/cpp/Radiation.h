@@ -10,10 +10,12 @@
 // RunSimulation uploads once, keeps the state on the device for the whole
 // loop and downloads I and the moments at the end.
 //
+// Frame output (writeData) follows the reference: lab-frame moments computed
+// on the GPU, CSV frames under OUTPUTDIR/<name>/Moments/ and Log.txt.
+//
 // Not part of this library (the methods exit with a message, like the
-// reference's ExitOnError): adaptive and geodesic streaming, initial data
-// given as moments (the flux-to-sigma lookup tables), lab-frame moments and
-// the CSV frame output.
+// reference's ExitOnError): adaptive and geodesic streaming and initial data
+// given as moments (the flux-to-sigma lookup tables).
 #ifndef RT2D_RADIATION_H
 #define RT2D_RADIATION_H
 #include "Config.hh"
@@ -37,6 +39,9 @@ class Radiation {
     void UploadI();
     void DownloadI();
     void DownloadMoments();
+    void DownloadMomentsLF();
+    void WriteLog(const std::string &directoryPath, const std::string &date, int timeSteps,
+                  double computeSeconds) const;
     void RequireSupportedStreaming() const;
 
   public:

@@ -5,7 +5,9 @@
 #include <cerrno>
 #include <cmath>
 #include <cstdlib>
+#include <filesystem>
 #include <iostream>
+#include <sstream>
 #include <string>
 #include <vector>
 #include "../csrc/rt_math.h"
@@ -42,6 +44,34 @@ template <int N>
 inline double IntegerPow(double a) { return a * IntegerPow<N - 1>(a); }
 template <>
 inline double IntegerPow<0>(double) { return 1; }
+
+// Output root of the frame files and Log.txt (src/ControlFlow.hh:18), relative
+// to the working directory, as in the reference.
+#define OUTPUTDIR (std::string) "../output/"
+
+// Fixed-point number text of the reference's Format (src/Utility.hh:511-540):
+// a leading space for n >= 0 (and for +0), none for negatives and -0; one
+// more space per missing digit below maxDigits (up to 3); `precision`
+// decimals.
+inline std::string Format(const double n, const int precision = 6, const bool noSpace = false, const int maxDigits = 1)
+{
+    std::string out;
+    if (!noSpace && !std::signbit(n)) out = " ";
+    if (maxDigits > 2 && std::abs(n) < 100) out += " ";
+    if (maxDigits > 1 && std::abs(n) < 10) out += " ";
+    std::ostringstream ss;
+    ss.precision(precision);
+    ss << std::fixed << n;
+    return out + ss.str();
+}
+
+// Creates `path` (relative to the working directory) with its parents
+// (src/Utility.hh:594-605); false if it existed or path is empty.
+inline bool CreateDirectory(const std::string &path)
+{
+    if (path.empty()) return false;
+    return std::filesystem::create_directories(std::filesystem::current_path() / path);
+}
 
 inline double MyAtan2(double y, double x) { return rtm::my_atan2(y, x); }
 inline double MySin(double x) { return rtm::my_sin(x); }

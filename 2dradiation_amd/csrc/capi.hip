@@ -289,6 +289,7 @@ void rt_destroy(rt_ctx *c)
     dfree(c->d_w); dfree(c->d_phi); dfree(c->d_cx); dfree(c->d_cy); dfree(c->d_Nc); dfree(c->d_Wc);
     dfree(c->d_basis); dfree(c->d_dirt); dfree(c->d_alpha); dfree(c->d_t11); dfree(c->d_t12);
     dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_coefT); dfree(c->d_excl); dfree(c->d_red);
+    dfree(c->d_frame); dfree(c->d_lf);
     dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
     dfree(c->d_lam[0]); dfree(c->d_lam[1]); dfree(c->d_grad[0]); dfree(c->d_grad[1]); dfree(c->d_pts); dfree(c->d_mask); dfree(c->d_f64);
     dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_mom); dfree(c->d_count);
@@ -541,6 +542,45 @@ int rt_get_moments(rt_ctx *c, double *E, double *Fx, double *Fy, double *Pxx, do
     double *dst[6] = {E, Fx, Fy, Pxx, Pxy, Pyy};
     for (int q = 0; q < 6; q++)
         if (dst[q]) RT_HIP(c, hipMemcpyAsync(dst[q], c->d_mom + (size_t)q * c->N, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_set_lab_frame(rt_ctx *c, const double *const tetrad[9], const double *gamma11, const double *gamma12,
+                     const double *gamma22)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    NEED(c, tetrad); NEED(c, gamma11); NEED(c, gamma12); NEED(c, gamma22);
+    const size_t N = c->N;
+    const double *src[12];
+    for (int k = 0; k < 9; k++) {
+        NEED(c, tetrad[k]);
+        src[k] = tetrad[k];
+    }
+    src[9] = gamma11;
+    src[10] = gamma12;
+    src[11] = gamma22;
+    TRY(dalloc(c, &c->d_frame, 12 * N * sizeof(double)));
+    TRY(dalloc(c, &c->d_lf, 7 * N * sizeof(double)));
+    for (int k = 0; k < 12; k++)
+        RT_HIP(c, hipMemcpyAsync(c->d_frame + (size_t)k * N, src[k], N * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    c->has_frame = true;
+    return RT_OK;
+}
+
+int rt_get_moments_lf(rt_ctx *c, double *E_LF, double *Fx_LF, double *Fy_LF, double *Pxx_LF, double *Pxy_LF,
+                      double *Pyy_LF, double *F_LF)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    if (!c->has_frame) return rt_fail(c, RT_ERR_STATE, "rt_set_lab_frame has not been called");
+    TRY(launch_moments_lf(c));
+    double *dst[7] = {E_LF, Fx_LF, Fy_LF, Pxx_LF, Pxy_LF, Pyy_LF, F_LF};
+    for (int q = 0; q < 7; q++)
+        if (dst[q])
+            RT_HIP(c, hipMemcpyAsync(dst[q], c->d_lf + (size_t)q * c->N, c->N * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream));
     RT_HIP(c, hipStreamSynchronize(c->stream));
     return RT_OK;
 }

@@ -727,6 +727,57 @@ int launch_ref_stream(rt_ctx *c, const void *in, void *out)
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }
+/* Lab-frame moments of ComputeMomentsLF (src/Radiation.cpp:270-301): the
+ * reference's loop order of TransformIFtoLF (src/SpecialMath.cpp:85-94) and
+ * Norm2 (:20-27), no contraction (this file is built with
+ * -ffp-contract=off). */
+__global__ void k_moments_lf(const double *__restrict__ mom, const double *__restrict__ frame,
+                             double *__restrict__ lf, long N, int nx, double startx, double starty, double dx,
+                             double dy, double r2)
+{
+    const long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (ij >= N) return;
+    const double x = startx + (double)(ij % nx) * dx, y = starty + (double)(ij / nx) * dy;
+    if (r2 >= 0.0 && x * x + y * y <= r2) {
+        for (int k = 0; k < 7; k++) lf[(size_t)k * N + ij] = 0.0;
+        return;
+    }
+    const double E = mom[ij], Fx = mom[N + ij], Fy = mom[2 * N + ij];
+    const double Pxx = mom[3 * N + ij], Pxy = mom[4 * N + ij], Pyy = mom[5 * N + ij];
+    const double T[3][3] = {{E, Fx, Fy}, {Fx, Pxx, Pxy}, {Fy, Pxy, Pyy}};
+    double e[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int A = 0; A < 3; A++) e[a][A] = frame[(size_t)(3 * a + A) * N + ij];
+    double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            for (int A = 0; A < 3; A++)
+                for (int B = 0; B < 3; B++) R[a][b] += T[A][B] * e[a][A] * e[b][B];
+    const double g[3][3] = {{0, 0, 0},
+                            {0, frame[9 * (size_t)N + ij], frame[10 * (size_t)N + ij]},
+                            {0, frame[10 * (size_t)N + ij], frame[11 * (size_t)N + ij]}};
+    const double v[3] = {0, R[0][1], R[0][2]};
+    double n2 = 0;
+    for (int i = 1; i < 3; i++)
+        for (int j = 1; j < 3; j++) n2 += g[i][j] * v[i] * v[j];
+    lf[ij] = R[0][0];
+    lf[N + ij] = R[0][1];
+    lf[2 * N + ij] = R[0][2];
+    lf[3 * N + ij] = R[1][1];
+    lf[4 * N + ij] = R[1][2];
+    lf[5 * N + ij] = R[2][2];
+    lf[6 * N + ij] = sqrt(fabs(n2));
+}
+
+int launch_moments_lf(rt_ctx *c)
+{
+    const rt_desc &d = c->desc;
+    k_moments_lf<<<nblk(c->N, 256), 256, 0, c->stream>>>(c->d_mom, c->d_frame, c->d_lf, c->N, d.nx, d.startx,
+                                                          d.starty, d.dx, d.dy, d.excision_r2);
+    RT_HIP(c, hipGetLastError());
+    return RT_OK;
+}
+
 int launch_ref_moments(rt_ctx *c, const void *in)
 {
     if (c->fp32)

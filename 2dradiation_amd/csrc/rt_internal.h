@@ -93,6 +93,10 @@ struct rt_ctx {
     /* curved fused stepping keeps the state as alpha^-3 I in d_I[cur] */
     bool scaled = false;
     double *d_mom = nullptr;      /* [6][N] */
+    /* lab frame: tetrad [9][N] and gamma_ll [3][N]; LF moments [7][N] */
+    double *d_frame = nullptr;
+    double *d_lf = nullptr;
+    bool has_frame = false;
     int32_t *d_count = nullptr;   /* [N] per-method collide cycle counts */
     unsigned long long *d_stats = nullptr; /* [2] sum, max */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -147,6 +151,7 @@ int launch_geodesic_tables(rt_ctx *c, const double *d_metric, double tol);
 int launch_departure_bound(rt_ctx *c, double *max_disp);
 int launch_fill(rt_ctx *c, double *p, size_t n, double v);
 int launch_coef_rel(rt_ctx *c);
+int launch_moments_lf(rt_ctx *c);
 int launch_adjoint_step(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, const RecordPtrs &rec);
 int launch_sample(rt_ctx *c, const void *I, int scaled, int npts, const double *px, const double *py, double *out);
 int launch_sample_adjoint(rt_ctx *c, void *lam, int scaled, int npts, const double *px, const double *py,

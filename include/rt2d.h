@@ -139,6 +139,20 @@ int rt_download_I(rt_ctx *ctx, double *I);
 /* Inertial-frame moments E,Fx,Fy,Pxx,Pxy,Pyy (Radiation::E..Pyy), each [N]. */
 int rt_get_moments(rt_ctx *ctx, double *E, double *Fx, double *Fy, double *Pxx, double *Pxy,
                    double *Pyy);
+/* Lab-frame moments (Radiation::ComputeMomentsLF, src/Radiation.cpp:270-301):
+ * the inertial-frame energy-momentum tensor [[E,Fx,Fy],[Fx,Pxx,Pxy],
+ * [Fy,Pxy,Pyy]] of the moments last computed (rt_compute_moments, or rt_step
+ * with want_moments) transformed with the tetrad, T_LF[a][b] =
+ * sum_AB T_IF[A][B] e[a][A] e[b][B] (TransformIFtoLF, src/SpecialMath.cpp:85-94),
+ * and F = sqrt(|gamma_ij F^i F^j|) (Norm2, src/SpecialMath.cpp:20-27).
+ * Cells inside the excision radius get 0.  rt_set_lab_frame takes the full
+ * tetrad e[a][A] (Metric::tetradAB_ul, 9 arrays [N] in the order 00, 01, 02,
+ * 10, ..., 22) and the spatial metric gamma_ll (11, 12, 22), each [N];
+ * rt_get_moments_lf writes 7 arrays [N]. */
+int rt_set_lab_frame(rt_ctx *ctx, const double *const tetrad[9], const double *gamma11,
+                     const double *gamma12, const double *gamma22);
+int rt_get_moments_lf(rt_ctx *ctx, double *E_LF, double *Fx_LF, double *Fy_LF, double *Pxx_LF,
+                      double *Pxy_LF, double *Pyy_LF, double *F_LF);
 /* Lambda-iteration statistics accumulated since the last reset: the maximum
  * per-cell cycle count and the sum of all per-cell counts
  * (Radiation::maxItterationCount / averageItterationCount numerator,

@@ -88,6 +88,7 @@ def lib(fast=False):
     for f in ("or_stream_flat", "or_stream_curved"):
         getattr(L, f).argtypes = [C.POINTER(Sim), dptr, dptr]
     L.or_moments.argtypes = [C.POINTER(Sim), dptr, dptr]
+    L.or_moments_lf.argtypes = [C.POINTER(Grid), dptr, dptr, dptr, d, dptr]
     L.or_collide.argtypes = [C.POINTER(Sim), dptr, dptr, dptr, C.c_void_p]
     L.or_num_threads.restype = i32
     L.or_set_threads.argtypes = [i32]
@@ -131,6 +132,17 @@ def metric(g, kind, m=1.0, a=0.0):
     out = np.zeros(NM * g.N)
     lib().or_metric_build(C.byref(g), kind, m, a, out)
     return out.reshape(NM, g.N)
+
+
+def moments_lf(g, mom, tetrad, gamma, r2=-1.0):
+    """ComputeMomentsLF (src/Radiation.cpp:270-301): mom [6][N], tetrad
+    [9][N] (e[a][A] row-major), gamma [3][N] (11, 12, 22) -> [7][N]
+    (E, Fx, Fy, Pxx, Pxy, Pyy, F) in the lab frame."""
+    out = np.zeros(7 * g.N)
+    lib().or_moments_lf(C.byref(g), np.ascontiguousarray(mom, dtype=float).reshape(-1),
+                        np.ascontiguousarray(tetrad, dtype=float).reshape(-1),
+                        np.ascontiguousarray(gamma, dtype=float).reshape(-1), r2, out)
+    return out.reshape(7, g.N)
 
 
 def geodesic_tables(g, M, bh_r2, tol=1e-4, nF=5):
@@ -201,6 +213,12 @@ class Problem:
                                    np.ascontiguousarray(lamT, dtype=np.float64), ge, gk)
         assert rc == 0
         return ge, gk
+
+    def moments(self, I):
+        """ComputeMomentsIF of a state: [6][N] (E, Fx, Fy, Pxx, Pxy, Pyy)."""
+        mom = np.zeros(6 * self.grid.N)
+        self.L.or_moments(C.byref(self.sim), np.ascontiguousarray(I, dtype=np.float64), mom)
+        return mom.reshape(6, self.grid.N)
 
     def run(self, I, nSteps, want_counts=False):
         """Advance I (AoS [N*nDir], float64) by nSteps; returns (I, moments, max, sum, counts)."""

@@ -677,6 +677,44 @@ void or_moments(const or_sim *s, const double *I, double *mom)
         E[ij] = e; Fx[ij] = fx; Fy[ij] = fy; Pxx[ij] = pxx; Pxy[ij] = pxy; Pyy[ij] = pyy;
     }
 }
+/* ComputeMomentsLF, Radiation.cpp:270-301: tetrad transform of the IF
+ * energy-momentum tensor (TransformIFtoLF, SpecialMath.cpp:85-94, same loop
+ * order) and F = sqrt(|Norm2(F, gamma_ll)|) (SpecialMath.cpp:20-27); cells
+ * with x^2 + y^2 <= r2 (InsideBH) get 0.  tetrad: [9][N] e[a][A] row-major,
+ * gamma: [3][N] (11, 12, 22), out: [7][N] (E, Fx, Fy, Pxx, Pxy, Pyy, F). */
+void or_moments_lf(const or_grid *g, const double *mom, const double *tetrad, const double *gamma, double r2,
+                   double *out)
+{
+    const long N = (long)g->nx * g->ny;
+    for (long ij = 0; ij < N; ij++) {
+        const double x = g->startx + (double)(ij % g->nx) * g->dx, y = g->starty + (double)(ij / g->nx) * g->dy;
+        if (or_inside_bh(r2, x, y)) {
+            for (int k = 0; k < 7; k++) out[k * N + ij] = 0.0;
+            continue;
+        }
+        const double E = mom[ij], Fx = mom[N + ij], Fy = mom[2 * N + ij];
+        const double T[3][3] = {{E, Fx, Fy}, {Fx, mom[3 * N + ij], mom[4 * N + ij]},
+                                {Fy, mom[4 * N + ij], mom[5 * N + ij]}};
+        double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                for (int A = 0; A < 3; A++)
+                    for (int B = 0; B < 3; B++)
+                        R[a][b] += T[A][B] * tetrad[(3 * a + A) * N + ij] * tetrad[(3 * b + B) * N + ij];
+        const double gm[3][3] = {{0, 0, 0}, {0, gamma[ij], gamma[N + ij]}, {0, gamma[N + ij], gamma[2 * N + ij]}};
+        const double v[3] = {0, R[0][1], R[0][2]};
+        double n2 = 0;
+        for (int i = 1; i < 3; i++)
+            for (int j = 1; j < 3; j++) n2 += gm[i][j] * v[i] * v[j];
+        out[ij] = R[0][0];
+        out[N + ij] = R[0][1];
+        out[2 * N + ij] = R[0][2];
+        out[3 * N + ij] = R[1][1];
+        out[4 * N + ij] = R[1][2];
+        out[5 * N + ij] = R[2][2];
+        out[6 * N + ij] = sqrt(fabs(n2));
+    }
+}
 /* Collide, Radiation.cpp:641-724.  Writes Inew at interior active cells,
  * updates the moment arrays in place, and records cycle counts. */
 void or_collide(const or_sim *s, const double *I, double *Inew, double *mom, int32_t *count)

@@ -96,6 +96,8 @@ int or_simulate(const or_sim *s, int32_t nSteps, double *I, double *moments,
 void or_stream_flat(const or_sim *s, const double *I, double *Inew);
 void or_stream_curved(const or_sim *s, const double *I, double *Inew);
 void or_moments(const or_sim *s, const double *I, double *mom /*[6][N]*/);
+void or_moments_lf(const or_grid *g, const double *mom /*[6][N]*/, const double *tetrad /*[9][N]*/,
+                   const double *gamma /*[3][N]*/, double r2, double *out /*[7][N]*/);
 /* Collide in place on the moment arrays; writes Inew for interior active cells. */
 void or_collide(const or_sim *s, const double *I, double *Inew, double *mom, int32_t *count);
 

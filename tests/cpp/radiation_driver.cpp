@@ -8,6 +8,9 @@
 //                                     CFL 0.5, 100 steps, Gaussian eta (sigma 0.1)
 //   radiation_driver curved out.bin   Schwarzschild 40x40 on [-4,4]^2, 16 directions,
 //                                     10 steps, eta = 1 on 3 < r < 3.5
+//   radiation_driver frames out.bin   Minkowski 40x40 on [-1,1]^2, 20 directions, 12 steps,
+//                                     Gaussian eta, writeData with writePeriod 5 dt: CSV
+//                                     frames and Log.txt under ../output/driver_frames/
 // out.bin: I (nxy*nDir doubles) then E, Fx, Fy (nxy doubles each).
 #include <cmath>
 #include <cstdio>
@@ -22,7 +25,8 @@ int main(int argc, char **argv)
         return 2;
     }
     const bool curved = std::strcmp(argv[1], "curved") == 0;
-    const size_t n = curved ? 40 : 100;
+    const bool frames = std::strcmp(argv[1], "frames") == 0;
+    const size_t n = (curved || frames) ? 40 : 100;
     const double L = curved ? 4.0 : 1.0;
     Grid grid(n, n, Coord(-L, -L), Coord(L, L));
     grid.SetCFL(0.5);
@@ -33,14 +37,14 @@ int main(int argc, char **argv)
     Stencil streamingStencil(5, 0, false);
 
     Config config = {
-        .name = curved ? "driver_curved" : "driver_flat",
-        .simTime = (curved ? 10 : 100) * grid.dt,
-        .writePeriod = 1e9,
-        .writeData = false,
+        .name = curved ? "driver_curved" : (frames ? "driver_frames" : "driver_flat"),
+        .simTime = (curved ? 10 : (frames ? 12 : 100)) * grid.dt,
+        .writePeriod = frames ? 5 * grid.dt : 1e9,
+        .writeData = frames,
         .printSetup = false,
         .printProgress = false,
         .printResults = true,
-        .saveInitialData = false,
+        .saveInitialData = frames,
         .streamingType = curved ? StreamingType::CurvedFixed : StreamingType::FlatFixed,
         .initialDataType = InitialDataType::Intensities,
     };

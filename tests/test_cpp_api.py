@@ -17,10 +17,10 @@ def test_driver_is_built():
     assert os.path.exists(DRIVER), "tests/cpp/_build/radiation_driver missing: run __graft_entry__.build()"
 
 
-def run_driver(mode, tmp_path):
+def run_driver(mode, tmp_path, cwd=None):
     out = str(tmp_path / f"{mode}.bin")
     print(f"[cpp-driver] {mode}", flush=True)
-    r = subprocess.run([DRIVER, mode, out], capture_output=True, text=True, timeout=300)
+    r = subprocess.run([DRIVER, mode, out], capture_output=True, text=True, timeout=300, cwd=cwd)
     assert r.returncode == 0, r.stdout + r.stderr
     return np.fromfile(out, dtype=np.float64), r.stdout
 
@@ -60,3 +60,52 @@ def test_radiation_curved_schwarzschild_matches_oracle(O, tmp_path):
                   fields={"eta": ((r > 3) & (r < 3.5)).astype(float)})
     Iref = P.run(np.zeros(N * D), 10)[0]
     assert rel_l2(data[:N * D], Iref) <= 1e-12
+
+
+def _format3(t):
+    """The reference's Format(t, 3) (src/Utility.hh:511-540) of a float time."""
+    return (" " if t >= 0 else "") + f"{float(np.float32(t)):.3f}"
+
+
+@pytest.mark.gpu
+def test_radiation_frames_csv_and_log(O, tmp_path):
+    """writeData: frames after steps 0 (saveInitialData), 5 and 10 (writePeriod
+    5 dt) and the last step, in the reference's file naming and layout, with the
+    lab-frame E, Fx, Fy, |F| of the oracle's state (Minkowski: identity
+    tetrad, so LF = IF); and Log.txt."""
+    build = tmp_path / "build"
+    build.mkdir()
+    run_driver("frames", tmp_path, cwd=str(build))
+    g = O.make_grid(40, 40, (-1, -1), (1, 1), 0.5)
+    N, D = g.N, 20
+    x = g.startx + (np.arange(N) % g.nx) * g.dx
+    y = g.starty + (np.arange(N) // g.nx) * g.dy
+    P = O.Problem(g, D, 0, O.metric(g, 0), fields={"eta": np.exp(-(x * x + y * y) / (2 * 0.1 ** 2))})
+    outdir = tmp_path / "output" / "driver_frames"
+    mdir = outdir / "Moments"
+    files = sorted(p.name for p in mdir.iterdir())
+    assert len(files) == 4, files
+    I = np.zeros(N * D)
+    done = 0
+    # timeSteps = ceil(simTime / dt) with simTime = 12 dt, as the reference
+    # computes it (src/Radiation.cpp:748): the rounding can make it 13
+    total = int(np.ceil((12 * g.dt) / g.dt))
+    for steps in (0, 5, 10, total):
+        if steps > done:
+            I = P.run(I, steps - done)[0]
+            done = steps
+        t = steps * g.dt
+        name = f"data{_format3(t)}t {g.nx}x {g.ny}y.csv"
+        assert name in files, (name, files)
+        lines = (mdir / name).read_text().splitlines()
+        assert lines[0] == f"#nx={g.nx}" and lines[1] == f"#ny={g.ny}"
+        assert lines[7] == "#x,y,r,g,b,a"
+        vals = np.array([[float(v) for v in ln.split(",")] for ln in lines[8:]])
+        assert vals.shape == (N, 6)
+        assert np.allclose(vals[:, 0], x, atol=1e-5) and np.allclose(vals[:, 1], y, atol=1e-5)
+        mom = O.moments_lf(g, P.moments(I), np.eye(3).reshape(9, 1).repeat(N, 1),
+                           np.stack([np.ones(N), np.zeros(N), np.ones(N)]))
+        for col, q in ((2, 0), (3, 1), (4, 2), (5, 6)):
+            assert np.allclose(vals[:, col], mom[q], rtol=2e-5, atol=1e-9 * max(1.0, np.abs(mom[q]).max())), (steps, q)
+    log = (outdir / "Log.txt").read_text()
+    assert "Spacetime: Minkowski" in log and f"Nt         = {total}" in log
