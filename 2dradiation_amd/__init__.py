@@ -74,6 +74,9 @@ def lib():
     L.rt_upload_I.argtypes = [vp, _dp]
     L.rt_download_I.argtypes = [vp, _dp]
     L.rt_get_moments.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.rt_build_flux_tables.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.POINTER(C.c_int32)]
+    L.rt_set_sources.argtypes = [vp, C.c_void_p, C.c_void_p]
     L.rt_set_lab_frame.argtypes = [vp, C.POINTER(C.POINTER(C.c_double)), _dp, _dp, _dp]
     L.rt_get_moments_lf.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.rt_iteration_stats.argtypes = [vp, C.POINTER(i32), C.POINTER(d)]
@@ -119,6 +122,16 @@ def build_stencil(n, tables=True):
     Wc = np.zeros(40 * n) if tables else None
     _check(lib().rt_build_stencil(n, w, phi, cx, cy, _ptr(Nc), _ptr(Wc)))
     return (w, phi, cx, cy, Nc, Wc) if tables else (w, phi, cx, cy)
+
+
+def build_flux_tables(n):
+    """Stencil(n).fluxToSigmaTable / fluxToNormalizationTable: (relative flux,
+    sigma, log-normalisation) arrays from the library's C++ Stencil class."""
+    count = C.c_int32()
+    _check(lib().rt_build_flux_tables(n, 0, None, None, None, C.byref(count)))
+    f, s_, z = (np.zeros(count.value) for _ in range(3))
+    _check(lib().rt_build_flux_tables(n, count.value, f.ctypes.data, s_.ctypes.data, z.ctypes.data, C.byref(count)))
+    return f, s_, z
 
 
 def build_metric(kind, m, a, grid):
@@ -221,6 +234,17 @@ class Context:
         arrs = [np.zeros(self.grid.N) for _ in range(6)]
         self._c(lib().rt_get_moments(self.h, *arrs))
         return np.stack(arrs)
+
+    def set_sources(self, is_source, I):
+        """Source cells re-imposed before every step (keepSourceNodesActive):
+        is_source [N] (nonzero marks a cell), I host AoS [N*nDir]; None
+        removes them."""
+        if is_source is None:
+            self._c(lib().rt_set_sources(self.h, None, None))
+            return
+        m = np.ascontiguousarray(is_source, dtype=np.uint8)
+        v = np.ascontiguousarray(I, dtype=np.float64)
+        self._c(lib().rt_set_sources(self.h, m.ctypes.data, v.ctypes.data))
 
     def set_lab_frame(self, tetrad, gamma):
         """Full tetrad e[a][A] as [9][N] (00, 01, ..., 22) and gamma_ll as

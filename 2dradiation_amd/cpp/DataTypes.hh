@@ -63,6 +63,40 @@ struct Tensor3x3 {
     }
 };
 
+// Sorted (input, output) table with clamped linear interpolation
+// (LookUpTable, src/DataTypes.hh:637-700): Add keeps the inputs increasing
+// and ignores an input already present; Evaluate returns the first/last
+// output outside the range, the stored output at an exact input, and the
+// lerp between the neighbours otherwise.
+template <class T>
+struct LookUpTable {
+    int size = 0;
+    std::vector<T> inputs;
+    std::vector<T> outputs;
+
+    void Add(T input, T output)
+    {
+        auto it = std::lower_bound(inputs.begin(), inputs.end(), input);
+        if (it != inputs.end() && !(input < *it)) return;
+        const auto k = it - inputs.begin();
+        inputs.insert(it, input);
+        outputs.insert(outputs.begin() + k, output);
+        size = (int)inputs.size();
+    }
+    T Evaluate(T x) const
+    {
+        if (size == 0) ExitOnError("Trying to read from empty look up table.");
+        if (x <= inputs[0] || size == 1) return outputs[0];
+        if (x >= inputs[size - 1]) return outputs[size - 1];
+        const auto hi = std::lower_bound(inputs.begin(), inputs.end(), x);
+        const size_t i1 = (size_t)(hi - inputs.begin());
+        if (inputs[i1] == x) return outputs[i1];
+        const size_t i0 = i1 - 1;
+        const T t = (x - inputs[i0]) / (inputs[i1] - inputs[i0]);
+        return outputs[i0] + t * (outputs[i1] - outputs[i0]);
+    }
+};
+
 // RotationMatrix (src/DataTypes.hh:620-635).
 struct RotationMatrix {
     double angle;

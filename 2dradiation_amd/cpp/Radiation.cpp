@@ -118,20 +118,67 @@ void Radiation::DownloadMoments()
     Check(rt_get_moments(ctx, E.data(), Fx.data(), Fy.data(), Pxx.data(), Pxy.data(), Pyy.data()));
 }
 
-// Reference: src/Radiation.cpp:124-189.  Initial intensities are copied for
-// the marked cells; initial moments need the reference's flux-to-sigma
-// lookup tables, which this library does not build.
+// Reference: src/Radiation.cpp:103-123 (InitialDataLFtoIF).  The lab-frame
+// tensor is transformed with the inverse tetrad; as in the reference only
+// its energy density is taken from the result, the flux from the lab-frame
+// tensor, and |F| > E is clipped to |F| = E.
+Tensor3 Radiation::InitialDataLFtoIF(size_t ij) const
+{
+    const double T[3][3] = {{initialE_LF[ij], initialFx_LF[ij], initialFy_LF[ij]},
+                            {initialFx_LF[ij], initialPxx_LF[ij], initialPxy_LF[ij]},
+                            {initialFy_LF[ij], initialPxy_LF[ij], initialPyy_LF[ij]}};
+    const Tensor3x3 e = metric.GetTetrad(ij).Invert();
+    double E_IF = 0;
+    for (int A = 0; A < 3; A++)
+        for (int B = 0; B < 3; B++) E_IF += T[A][B] * e[{0, A}] * e[{0, B}];
+    double Fx_IF = T[0][1], Fy_IF = T[0][2];
+    const double F_IF = Tensor2(Fx_IF, Fy_IF).EuklNorm();
+    if (F_IF > E_IF) {
+        Fx_IF *= E_IF / F_IF;
+        Fy_IF *= E_IF / F_IF;
+    }
+    return Tensor3(E_IF, Fx_IF, Fy_IF);
+}
+
+// Reference: src/Radiation.cpp:179-185 (NormalizationOverwrite).
+double Radiation::NormalizationOverwrite() const
+{
+    double n = 0;
+    for (size_t d = 0; d < stencil.nDir; d++) n += stencil.W(d) * std::exp(sigmaOverwrite * Tensor2::Dot(Tensor2(1, 0), stencil.C(d)));
+    return std::log(n);
+}
+
+// Reference: src/Radiation.cpp:124-178.  For the marked cells: intensities as
+// given (InitialDataType::Intensities), or (Moments) a von Mises distribution
+// I_d = E exp(sigma n.C_d - normalization) around the inertial-frame flux
+// direction n, with sigma and the normalization from the stencil's
+// flux-to-sigma tables at |F|/E (or sigmaOverwrite).  Fixed streaming keeps
+// the rotation angle 0.
 void Radiation::LoadInitialData()
 {
-    bool any = false;
-    for (size_t ij = 0; ij < grid.nxy; ij++) any = any || isInitialGridPoint[ij];
-    if (!any) return;
-    if (config.initialDataType == InitialDataType::Moments)
-        ExitOnError("initial data given as moments is not part of this library; use InitialDataType::Intensities");
+    const double normalizationOverwrite = (0 <= sigmaOverwrite) ? NormalizationOverwrite() : 0.0;
     for (size_t ij = 0; ij < grid.nxy; ij++) {
         if (!isInitialGridPoint[ij]) continue;
         rotationAngle[ij] = 0;
-        for (size_t d = 0; d < stencil.nDir; d++) I[Index(ij, d)] = initialI[Index(ij, d)];
+        if (config.initialDataType == InitialDataType::Intensities) {
+            for (size_t d = 0; d < stencil.nDir; d++) I[Index(ij, d)] = initialI[Index(ij, d)];
+            continue;
+        }
+        const Tensor3 m = InitialDataLFtoIF(ij);
+        const double E_IF = m[0];
+        const Tensor2 F(m[1], m[2]);
+        const double Fn = F.EuklNorm();
+        const double relativeF = Fn / E_IF;
+        Tensor2 dir(m[1] / Fn, m[2] / Fn);
+        if (Fn < MIN_FLUX_NORM) dir = Tensor2(1, 0);
+        double sigma = stencil.fluxToSigmaTable.Evaluate(relativeF);
+        double normalization = stencil.fluxToNormalizationTable.Evaluate(relativeF);
+        if (0 <= sigmaOverwrite) {
+            sigma = sigmaOverwrite;
+            normalization = normalizationOverwrite;
+        }
+        for (size_t d = 0; d < stencil.nDir; d++)
+            I[Index(ij, d)] = E_IF * std::exp(sigma * Tensor2::Dot(dir, stencil.C(d)) - normalization);
     }
 }
 
@@ -300,6 +347,13 @@ void Radiation::RunSimulation()
 
     UploadFields();
     UploadI();
+    if (config.keepSourceNodesActive) {
+        /* the source cells' initial intensities are re-imposed on the GPU
+         * before every step */
+        std::vector<uint8_t> src(grid.nxy);
+        for (size_t ij = 0; ij < grid.nxy; ij++) src[ij] = isInitialGridPoint[ij] ? 1 : 0;
+        Check(rt_set_sources(ctx, src.data(), I.data()));
+    }
     Check(rt_reset_iteration_stats(ctx));
     double currentTime = config.t0, timeSinceLastFrame = 0, computeSeconds = 0;
     auto frame = [&]() {
@@ -310,7 +364,7 @@ void Radiation::RunSimulation()
         timeSinceLastFrame = 0;
     };
     if (config.writeData && config.saveInitialData) frame();
-    const bool perStep = config.updateFourierHarmonics || config.keepSourceNodesActive;
+    const bool perStep = config.updateFourierHarmonics;
     for (int n = 0; n < timeSteps;) {
         /* the steps up to the next frame, or to the end */
         int k = 0;
@@ -327,14 +381,9 @@ void Radiation::RunSimulation()
         if (!perStep) {
             Check(rt_step(ctx, k, last));
         } else {
-            // per-step hooks of the reference loop (src/Radiation.cpp:790-793)
+            // per-step table rebuild of the reference loop (src/Radiation.cpp:790-791)
             for (int s = 0; s < k; s++) {
-                if (config.updateFourierHarmonics) UpdateFourierCoefficients();
-                if (config.keepSourceNodesActive) {
-                    DownloadI();
-                    LoadInitialData();
-                    UploadI();
-                }
+                UpdateFourierCoefficients();
                 Check(rt_step(ctx, 1, last && s == k - 1 ? 1 : 0));
             }
         }
@@ -360,4 +409,5 @@ void Radiation::RunSimulation()
         std::cout << "Benchmark: " << grid.nxy * timeSteps / (1e6 * computeSeconds) << "MLUPS" << std::endl;
     }
     if (config.writeData) WriteLog(directoryPath, date, timeSteps, computeSeconds);
+    if (config.keepSourceNodesActive) Check(rt_set_sources(ctx, nullptr, nullptr));
 }

@@ -14,8 +14,7 @@
 // on the GPU, CSV frames under OUTPUTDIR/<name>/Moments/ and Log.txt.
 //
 // Not part of this library (the methods exit with a message, like the
-// reference's ExitOnError): adaptive and geodesic streaming and initial data
-// given as moments (the flux-to-sigma lookup tables).
+// reference's ExitOnError): adaptive and geodesic streaming.
 #ifndef RT2D_RADIATION_H
 #define RT2D_RADIATION_H
 #include "Config.hh"
@@ -43,6 +42,8 @@ class Radiation {
     void WriteLog(const std::string &directoryPath, const std::string &date, int timeSteps,
                   double computeSeconds) const;
     void RequireSupportedStreaming() const;
+    Tensor3 InitialDataLFtoIF(size_t ij) const;
+    double NormalizationOverwrite() const;
 
   public:
     Grid &grid;

@@ -67,5 +67,63 @@ Stencil::Stencil(size_t nOrder_, int nGhost_, bool isIntensityStencil)
         cx[d] = MyCos(phi[d]);
         cy[d] = MySin(phi[d]);
     }
-    if (isIntensityStencil) interpolationGrid = InterpolationGrid(10 * nOrder, *this);
+    if (isIntensityStencil) {
+        interpolationGrid = InterpolationGrid(10 * nOrder, *this);
+        PopulateLookUpTable();
+        sigmaMax = fluxToSigmaTable.outputs[fluxToSigmaTable.size - 1];
+        relativeFluxMax = fluxToSigmaTable.inputs[fluxToSigmaTable.size - 1];
+    }
+}
+
+// Stencil::PopulateLookUpTable (reference src/Stencil.cpp:378-455): walk the
+// von Mises concentration sigma of a unit-energy distribution along x up in
+// steps of 10^-refinement.  A sigma is accepted, and (relative flux, sigma)
+// and (relative flux, log-normalisation) enter the tables, while the mean
+// relative error of the cubic angular interpolation at the fine angles within
+// deltaPhi/2 of the flux direction stays <= maxInterpolationError; otherwise
+// the step is undone and refined, until the refinement reaches 8.
+void Stencil::PopulateLookUpTable()
+{
+    const Tensor2 F(1, 0);
+    std::vector<double> I(nDir);
+    double sigma = 0;
+    int refinement = 1;
+    while (true) {
+        double normalization = 0;
+        for (size_t d = 0; d < nDir; d++) normalization += W(d) * std::exp(sigma * Tensor2::Dot(F, C(d)));
+        normalization = std::log(normalization);
+
+        double Fx = 0, Fy = 0;
+        for (size_t d = 0; d < nDir; d++) {
+            I[d] = std::exp(sigma * Tensor2::Dot(F, C(d)) - normalization);
+            const double c = W(d) * I[d];
+            Fx += c * Cx(d);
+            Fy += c * Cy(d);
+        }
+        const double relativeF = Tensor2(Fx, Fy).EuklNorm();
+
+        double error = 0;
+        int count = 0;
+        for (size_t g = 0; g < interpolationGrid.nGrid; g++) {
+            if (MyAcos(Tensor2::Dot(F, interpolationGrid.C(g))) > deltaPhi / 2.0) continue;
+            const double exact = std::exp(sigma * Tensor2::Dot(F, interpolationGrid.C(g)) - normalization);
+            double cubic = 0;
+            for (int p = 0; p < 4; p++)
+                cubic += interpolationGrid.neighbourWeightsCubic[g][p] * I[interpolationGrid.neighbourIndexesCubic[g][p]];
+            error += std::abs((exact - cubic) / exact);
+            count++;
+        }
+        error /= count;
+
+        if (error > maxInterpolationError) {
+            sigma -= std::pow(10, -refinement);
+            refinement++;
+            sigma += std::pow(10, -refinement);
+            if (refinement >= 8) break;
+        } else {
+            fluxToSigmaTable.Add(relativeF, sigma);
+            fluxToNormalizationTable.Add(relativeF, normalization);
+            sigma += std::pow(10, -refinement);
+        }
+    }
 }

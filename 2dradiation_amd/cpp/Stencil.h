@@ -1,7 +1,7 @@
-// Velocity stencil and its cubic angular interpolation grid
-// (reference: src/Stencil.h, src/Stencil.cpp:4-162, 177-297).  Ghost
-// directions (adaptive streaming) and the initial-data lookup tables are not
-// part of this library: nGhost must be 0.
+// Velocity stencil, its cubic angular interpolation grid and the
+// flux-to-sigma lookup tables of the von Mises initial data (reference:
+// src/Stencil.h, src/Stencil.cpp:4-162, 177-297, 378-455).  Ghost directions
+// (adaptive streaming) are not part of this library: nGhost must be 0.
 #ifndef RT2D_STENCIL_H
 #define RT2D_STENCIL_H
 #include <array>
@@ -18,12 +18,23 @@ struct InterpolationGrid {
     InterpolationGrid(size_t nGrid, const Stencil &stencil);
     double d(double phi) const;
     double Phi(size_t d) const { return 2.0 * rtm::kPi * d / nGrid; }
+    double Cx(size_t d) const { return MyCos(Phi(d)); }
+    double Cy(size_t d) const { return MySin(Phi(d)); }
+    Tensor2 C(size_t d) const { return Tensor2(Cx(d), Cy(d)); }
 };
 
 struct Stencil {
     std::string name;
     size_t nDir, nGhost, nOrder, nCoefficients;
     InterpolationGrid interpolationGrid;
+    // relative flux |F|/E of a von Mises distribution -> its concentration
+    // sigma and log-normalisation (intensity stencils only)
+    LookUpTable<double> fluxToSigmaTable;
+    LookUpTable<double> fluxToNormalizationTable;
+    double sigmaMax = 0;
+    double relativeFluxMax = 0;
+    static constexpr double deltaPhi = rtm::kPi / 8.0 + 1e-8;
+    static constexpr double maxInterpolationError = 0.01;
 
   private:
     DoubleBuffer w, phi, cx, cy;
@@ -39,5 +50,6 @@ struct Stencil {
     const double *PhiData() const { return phi.data(); }
     const double *CxData() const { return cx.data(); }
     const double *CyData() const { return cy.data(); }
+    void PopulateLookUpTable();
 };
 #endif

@@ -3,6 +3,7 @@
 #ifndef RT2D_UTILITY_HH
 #define RT2D_UTILITY_HH
 #include <cerrno>
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <filesystem>
@@ -74,6 +75,22 @@ inline bool CreateDirectory(const std::string &path)
 }
 
 inline double MyAtan2(double y, double x) { return rtm::my_atan2(y, x); }
+// Arc cosine of the reference (MyAcos, src/Utility.hh:453-471 with
+// APPROXIMATION_ORDER 9): the Hastings (1955) 4th-order arcsine
+// asin(x) = pi/2 - sqrt(1-x) (a0 + a1 x + ... + a4 x^4) for x in [0, 1], odd
+// in x; the argument and the result pass through float, as there.
+inline double MyAsinHastings4(double x)
+{
+    const double s = (x > 0) - (x < 0);
+    x = std::abs(x);
+    const double p = 1.57078786 + x * (-0.21412453 + x * (0.08466649 + x * (-0.03575663 + x * 0.00864884)));
+    return s * (rtm::kPi / 2 - std::sqrt(1.0 - x) * p);
+}
+inline float MyAcos(float x)
+{
+    return x >= 0 ? (float)(rtm::kPi / 2 - MyAsinHastings4(x)) : (float)(rtm::kPi / 2 + MyAsinHastings4(-x));
+}
+inline double MyAcos(double x) { return MyAcos((float)x); }
 inline double MySin(double x) { return rtm::my_sin(x); }
 inline double MyCos(double x) { return rtm::my_cos(x); }
 inline double MyAtan(double z) { return rtm::hastings_atan(z); }

@@ -289,7 +289,7 @@ void rt_destroy(rt_ctx *c)
     dfree(c->d_w); dfree(c->d_phi); dfree(c->d_cx); dfree(c->d_cy); dfree(c->d_Nc); dfree(c->d_Wc);
     dfree(c->d_basis); dfree(c->d_dirt); dfree(c->d_alpha); dfree(c->d_t11); dfree(c->d_t12);
     dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_coefT); dfree(c->d_excl); dfree(c->d_red);
-    dfree(c->d_frame); dfree(c->d_lf);
+    dfree(c->d_frame); dfree(c->d_lf); dfree(c->d_src_idx); dfree(c->d_src_I);
     dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
     dfree(c->d_lam[0]); dfree(c->d_lam[1]); dfree(c->d_grad[0]); dfree(c->d_grad[1]); dfree(c->d_pts); dfree(c->d_mask); dfree(c->d_f64);
     dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_mom); dfree(c->d_count);
@@ -570,6 +570,43 @@ int rt_set_lab_frame(rt_ctx *c, const double *const tetrad[9], const double *gam
     return RT_OK;
 }
 
+int rt_set_sources(rt_ctx *c, const uint8_t *isSource, const double *I)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    c->ck_valid = false;
+    if (!isSource) {
+        c->nsrc = 0;
+        return RT_OK;
+    }
+    NEED(c, I);
+    const size_t N = c->N, D = c->D;
+    std::vector<int32_t> idx;
+    for (size_t ij = 0; ij < N; ij++)
+        if (isSource[ij]) idx.push_back((int32_t)ij);
+    const int n = (int)idx.size();
+    std::vector<double> soa((size_t)n * D);
+    for (int k = 0; k < n; k++)
+        for (size_t d = 0; d < D; d++) {
+            const double v = I[(size_t)idx[k] * D + d];
+            if (!std::isfinite(v)) return rt_fail(c, RT_ERR_NONFINITE, "source intensities contain NaN/Inf");
+            soa[d * n + k] = v;
+        }
+    if (n > c->src_cap) {
+        dfree(c->d_src_idx);
+        dfree(c->d_src_I);
+        TRY(dalloc(c, &c->d_src_idx, (size_t)n * sizeof(int32_t)));
+        TRY(dalloc(c, &c->d_src_I, soa.size() * sizeof(double)));
+        c->src_cap = n;
+    }
+    if (n > 0) {
+        RT_HIP(c, hipMemcpyAsync(c->d_src_idx, idx.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        RT_HIP(c, hipMemcpyAsync(c->d_src_I, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        RT_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    c->nsrc = n;
+    return RT_OK;
+}
+
 int rt_get_moments_lf(rt_ctx *c, double *E_LF, double *Fx_LF, double *Fy_LF, double *Pxx_LF, double *Pxy_LF,
                       double *Pyy_LF, double *F_LF)
 {
@@ -665,12 +702,14 @@ int rt_step(rt_ctx *c, int32_t nSteps, int32_t want_moments)
         TRY(prepare_fused(c));
         if (want_moments) TRY(launch_fill(c, c->d_mom, 6 * (size_t)c->N, 0.0));
         for (int n = 0; n < nSteps; n++) {
+            TRY(launch_restore_sources(c, c->d_I[c->cur], c->scaled));
             TRY(launch_fused_step(c, c->d_I[c->cur], c->d_I[1 - c->cur], want_moments && n == nSteps - 1));
             c->cur = 1 - c->cur;
         }
     } else {
         TRY(ensure_unscaled(c));
         for (int n = 0; n < nSteps; n++) {
+            TRY(launch_restore_sources(c, c->d_I[c->cur], false));
             TRY(launch_ref_stream(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
             c->cur = 1 - c->cur;
             TRY(launch_ref_moments(c, c->d_I[c->cur]));
@@ -727,6 +766,7 @@ static int check_adjoint_ready(rt_ctx *c)
 {
     TRY(check_ready(c));
     if (c->any_u) return rt_fail(c, RT_ERR_UNSUPPORTED, "the adjoint needs zero fluid velocity");
+    if (c->nsrc > 0) return rt_fail(c, RT_ERR_UNSUPPORTED, "the adjoint does not take source cells (rt_set_sources)");
     if (!fused_supported(c)) return rt_fail(c, RT_ERR_UNSUPPORTED, "the adjoint needs the fused step (stencil or time step not covered)");
     return RT_OK;
 }
@@ -870,6 +910,22 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
     RT_HIP(c, hipMemcpyAsync(grad_eta, c->d_grad[0], N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     RT_HIP(c, hipMemcpyAsync(grad_kappa0, c->d_grad[1], N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_build_flux_tables(int32_t nOrder, int32_t capacity, double *relativeFlux, double *sigma, double *normalization,
+                         int32_t *count)
+{
+    if (nOrder < 4 || nOrder > 4096 || capacity < 0 || !count || (capacity > 0 && (!relativeFlux || !sigma || !normalization)))
+        return rt_fail(nullptr, RT_ERR_ARG, "rt_build_flux_tables: bad arguments");
+    Stencil s(nOrder, 0, true);
+    const int n = s.fluxToSigmaTable.size;
+    *count = n;
+    for (int k = 0; k < n && k < capacity; k++) {
+        relativeFlux[k] = s.fluxToSigmaTable.inputs[k];
+        sigma[k] = s.fluxToSigmaTable.outputs[k];
+        normalization[k] = s.fluxToNormalizationTable.outputs[k];
+    }
     return RT_OK;
 }
 

@@ -442,6 +442,7 @@ int launch_all(rt_batch *b, int parity)
 int batch_prepare(rt_batch *b)
 {
     for (rt_ctx *c : b->ctx) {
+        if (c->nsrc > 0) return bfail(b, RT_ERR_UNSUPPORTED, "batch members cannot have source cells (rt_set_sources)");
         if (int r = prepare_fused(c)) return bfail(b, r, std::string("batch member: ") + c->err);
         c->ck_valid = false;
         BHIP(b, hipStreamSynchronize(c->stream));

@@ -769,6 +769,40 @@ __global__ void k_moments_lf(const double *__restrict__ mom, const double *__res
     lf[6 * N + ij] = sqrt(fabs(n2));
 }
 
+/* keepSourceNodesActive (src/Radiation.cpp:790-791, LoadInitialData at the
+ * start of every step): the source cells get their initial intensities back;
+ * a state held as alpha^-3 I gets them scaled as k_scale_state does. */
+template <typename T>
+__global__ void k_restore_sources(T *__restrict__ I, const int32_t *__restrict__ idx, const double *__restrict__ src,
+                                  int nsrc, int D, long N, const T *__restrict__ ia3)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= (long)nsrc * D) return;
+    const int k = (int)(i % nsrc), d = (int)(i / nsrc);
+    const long ij = idx[k];
+    T x = (T)src[i];
+    if (ia3) {
+        const T s = ia3[ij];
+        if (s > T(0) && s < T(INFINITY)) x = x * s;
+    }
+    I[(size_t)d * N + ij] = x;
+}
+
+int launch_restore_sources(rt_ctx *c, void *state, bool scaled)
+{
+    if (c->nsrc == 0) return RT_OK;
+    const long n = (long)c->nsrc * c->D;
+    const size_t off = (size_t)CF_IA3 * c->N;
+    if (c->fp32)
+        k_restore_sources<float><<<nblk(n, 256), 256, 0, c->stream>>>(
+            (float *)state, c->d_src_idx, c->d_src_I, c->nsrc, c->D, c->N, scaled ? (const float *)c->d_cell + off : nullptr);
+    else
+        k_restore_sources<double><<<nblk(n, 256), 256, 0, c->stream>>>(
+            (double *)state, c->d_src_idx, c->d_src_I, c->nsrc, c->D, c->N, scaled ? (const double *)c->d_cell + off : nullptr);
+    RT_HIP(c, hipGetLastError());
+    return RT_OK;
+}
+
 int launch_moments_lf(rt_ctx *c)
 {
     const rt_desc &d = c->desc;

@@ -97,6 +97,11 @@ struct rt_ctx {
     double *d_frame = nullptr;
     double *d_lf = nullptr;
     bool has_frame = false;
+    /* source cells re-imposed before every step: cell indices and their
+     * intensities SoA [nDir][nsrc], FP64 */
+    int32_t *d_src_idx = nullptr;
+    double *d_src_I = nullptr;
+    int nsrc = 0, src_cap = 0;
     int32_t *d_count = nullptr;   /* [N] per-method collide cycle counts */
     unsigned long long *d_stats = nullptr; /* [2] sum, max */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -152,6 +157,7 @@ int launch_departure_bound(rt_ctx *c, double *max_disp);
 int launch_fill(rt_ctx *c, double *p, size_t n, double v);
 int launch_coef_rel(rt_ctx *c);
 int launch_moments_lf(rt_ctx *c);
+int launch_restore_sources(rt_ctx *c, void *state, bool scaled);
 int launch_adjoint_step(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, const RecordPtrs &rec);
 int launch_sample(rt_ctx *c, const void *I, int scaled, int npts, const double *px, const double *py, double *out);
 int launch_sample_adjoint(rt_ctx *c, void *lam, int scaled, int npts, const double *px, const double *py,

@@ -139,6 +139,12 @@ int rt_download_I(rt_ctx *ctx, double *I);
 /* Inertial-frame moments E,Fx,Fy,Pxx,Pxy,Pyy (Radiation::E..Pyy), each [N]. */
 int rt_get_moments(rt_ctx *ctx, double *E, double *Fx, double *Fy, double *Pxx, double *Pxy,
                    double *Pyy);
+/* Source cells (Config::keepSourceNodesActive: LoadInitialData at the start
+ * of every step, src/Radiation.cpp:790-791 and :124-178): rt_step puts the
+ * given intensities (host AoS [N*nDir], read at cells with isSource[ij] != 0)
+ * back into those cells before each step.  isSource == NULL removes them.
+ * The adjoint and batches do not take source cells (RT_ERR_UNSUPPORTED). */
+int rt_set_sources(rt_ctx *ctx, const uint8_t *isSource, const double *I);
 /* Lab-frame moments (Radiation::ComputeMomentsLF, src/Radiation.cpp:270-301):
  * the inertial-frame energy-momentum tensor [[E,Fx,Fy],[Fx,Pxx,Pxy],
  * [Fy,Pxy,Pyy]] of the moments last computed (rt_compute_moments, or rt_step
@@ -233,6 +239,12 @@ int rt_time_steps(rt_ctx *ctx, int32_t nSteps, float *ms);
 /* Stencil(nOrder, 0, isIntensityStencil) (src/Stencil.cpp:177-297):
  * w, phi, cx, cy get nOrder entries; if Nc/Wc are non-null they get the
  * InterpolationGrid(10*nOrder) cubic tables (nGrid = 10*nOrder, *4 each). */
+/* The flux-to-sigma tables of Stencil(nOrder) (Stencil::PopulateLookUpTable,
+ * src/Stencil.cpp:378-455): relative flux |F|/E of a von Mises distribution,
+ * its concentration sigma and log-normalisation, increasing in the flux.
+ * *count receives the number of entries; at most `capacity` are written. */
+int rt_build_flux_tables(int32_t nOrder, int32_t capacity, double *relativeFlux, double *sigma,
+                         double *normalization, int32_t *count);
 int rt_build_stencil(int32_t nOrder, double *w, double *phi, double *cx, double *cy,
                      int32_t *Nc, double *Wc);
 /* Metric arrays of Minkowski (kind 0), SchwarzSchild (1) or KerrSchild (2)

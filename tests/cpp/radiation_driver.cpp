@@ -11,6 +11,8 @@
 //   radiation_driver frames out.bin   Minkowski 40x40 on [-1,1]^2, 20 directions, 12 steps,
 //                                     Gaussian eta, writeData with writePeriod 5 dt: CSV
 //                                     frames and Log.txt under ../output/driver_frames/
+//   radiation_driver vonmises out.bin Minkowski 20x20 on [-1,1]^2, 16 directions, no steps:
+//                                     initial data given as lab-frame moments at three cells
 // out.bin: I (nxy*nDir doubles) then E, Fx, Fy (nxy doubles each).
 #include <cmath>
 #include <cstdio>
@@ -26,19 +28,20 @@ int main(int argc, char **argv)
     }
     const bool curved = std::strcmp(argv[1], "curved") == 0;
     const bool frames = std::strcmp(argv[1], "frames") == 0;
-    const size_t n = (curved || frames) ? 40 : 100;
+    const bool vonmises = std::strcmp(argv[1], "vonmises") == 0;
+    const size_t n = vonmises ? 20 : ((curved || frames) ? 40 : 100);
     const double L = curved ? 4.0 : 1.0;
     Grid grid(n, n, Coord(-L, -L), Coord(L, L));
     grid.SetCFL(0.5);
     Minkowski flat(grid, 1.0, 0.0);
     SchwarzSchild bh(grid, 1.0, 0.0);
     Metric &metric = curved ? (Metric &)bh : (Metric &)flat;
-    Stencil stencil(curved ? 16 : 20);
+    Stencil stencil((curved || vonmises) ? 16 : 20);
     Stencil streamingStencil(5, 0, false);
 
     Config config = {
         .name = curved ? "driver_curved" : (frames ? "driver_frames" : "driver_flat"),
-        .simTime = (curved ? 10 : (frames ? 12 : 100)) * grid.dt,
+        .simTime = (vonmises ? 0 : (curved ? 10 : (frames ? 12 : 100))) * grid.dt,
         .writePeriod = frames ? 5 * grid.dt : 1e9,
         .writeData = frames,
         .printSetup = false,
@@ -46,9 +49,21 @@ int main(int argc, char **argv)
         .printResults = true,
         .saveInitialData = frames,
         .streamingType = curved ? StreamingType::CurvedFixed : StreamingType::FlatFixed,
-        .initialDataType = InitialDataType::Intensities,
+        .initialDataType = vonmises ? InitialDataType::Moments : InitialDataType::Intensities,
     };
     Radiation radiation(metric, stencil, streamingStencil, config);
+    if (vonmises) {
+        /* (E, Fx, Fy) in the lab frame at three cells: relative flux 0.5, an
+         * isotropic cell, and |F| > E (clipped to |F| = E) */
+        const size_t cells[3] = {grid.Index(5, 5), grid.Index(10, 12), grid.Index(15, 7)};
+        const double mom[3][3] = {{2.0, 0.6, 0.8}, {1.5, 0.0, 0.0}, {1.0, -3.0, 4.0}};
+        for (int c = 0; c < 3; c++) {
+            radiation.isInitialGridPoint[cells[c]] = true;
+            radiation.initialE_LF[cells[c]] = mom[c][0];
+            radiation.initialFx_LF[cells[c]] = mom[c][1];
+            radiation.initialFy_LF[cells[c]] = mom[c][2];
+        }
+    }
     for (size_t j = 0; j < grid.ny; j++)
         for (size_t i = 0; i < grid.nx; i++) {
             const size_t ij = grid.Index(i, j);

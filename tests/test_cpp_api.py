@@ -109,3 +109,37 @@ def test_radiation_frames_csv_and_log(O, tmp_path):
             assert np.allclose(vals[:, col], mom[q], rtol=2e-5, atol=1e-9 * max(1.0, np.abs(mom[q]).max())), (steps, q)
     log = (outdir / "Log.txt").read_text()
     assert "Spacetime: Minkowski" in log and f"Nt         = {total}" in log
+
+
+@pytest.mark.gpu
+def test_radiation_initial_moments_von_mises(rt, tmp_path):
+    """InitialDataType::Moments (src/Radiation.cpp:103-178) in Minkowski
+    (identity tetrad): I_d = E exp(sigma n.C_d - z) with n the flux direction
+    and (sigma, z) interpolated in the stencil's flux tables at |F|/E;
+    |F| > E is clipped to |F| = E; zero flux gives direction (1, 0)."""
+    data, _ = run_driver("vonmises", tmp_path)
+    n, D = 22, 16
+    I = data[:n * n * D].reshape(n * n, D)
+    f, s, z = rt.build_flux_tables(D)
+    w, phi, cx, cy = rt.build_stencil(D)[:4]
+    cases = [((5, 5), 2.0, 0.6, 0.8), ((10, 12), 1.5, 0.0, 0.0), ((15, 7), 1.0, -3.0, 4.0)]
+    marked = set()
+    for (i, j), E, Fx, Fy in cases:
+        ij = i + j * n
+        marked.add(ij)
+        F = np.hypot(Fx, Fy)
+        if F > E:
+            Fx, Fy, F = Fx * E / F, Fy * E / F, E
+        nx_, ny_ = (Fx / F, Fy / F) if F >= 1e-16 else (1.0, 0.0)
+        sig = np.interp(F / E, f, s)
+        zz = np.interp(F / E, f, z)
+        want = E * np.exp(sig * (nx_ * cx + ny_ * cy) - zz)
+        assert np.allclose(I[ij], want, rtol=1e-12, atol=0), (i, j)
+        # energy: E up to the stencil's lack of rotation invariance (the
+        # normalisation is fitted along x), flux along n
+        assert abs(np.sum(w * I[ij]) - E) < 2e-3 * E
+        if F > 0:
+            Fv = np.array([np.sum(w * I[ij] * cx), np.sum(w * I[ij] * cy)])
+            assert abs(np.dot(Fv, [-ny_, nx_])) < 1e-3 * np.linalg.norm(Fv)
+    others = [k for k in range(n * n) if k not in marked]
+    assert np.all(I[others] == 0)
