@@ -77,6 +77,7 @@ def lib():
     L.rt_build_flux_tables.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.POINTER(C.c_int32)]
     L.rt_set_sources.argtypes = [vp, C.c_void_p, C.c_void_p]
+    L.rt_rebuild_fourier_tables.argtypes = [vp, C.c_double]
     L.rt_set_lab_frame.argtypes = [vp, C.POINTER(C.POINTER(C.c_double)), _dp, _dp, _dp]
     L.rt_get_moments_lf.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.rt_iteration_stats.argtypes = [vp, C.POINTER(i32), C.POINTER(d)]
@@ -234,6 +235,10 @@ class Context:
         arrs = [np.zeros(self.grid.N) for _ in range(6)]
         self._c(lib().rt_get_moments(self.h, *arrs))
         return np.stack(arrs)
+
+    def rebuild_fourier_tables(self, tol=1e-4):
+        """RK45 tables again from the metric block kept on the device."""
+        self._c(lib().rt_rebuild_fourier_tables(self.h, tol))
 
     def set_sources(self, is_source, I):
         """Source cells re-imposed before every step (keepSourceNodesActive):

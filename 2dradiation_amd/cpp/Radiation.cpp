@@ -381,9 +381,11 @@ void Radiation::RunSimulation()
         if (!perStep) {
             Check(rt_step(ctx, k, last));
         } else {
-            // per-step table rebuild of the reference loop (src/Radiation.cpp:790-791)
+            // per-step table rebuild of the reference loop (src/Radiation.cpp:790-791),
+            // from the metric block kept on the device
             for (int s = 0; s < k; s++) {
-                UpdateFourierCoefficients();
+                if (config.streamingType == StreamingType::CurvedFixed)
+                    Check(rt_rebuild_fourier_tables(ctx, config.rk45Tolerance));
                 Check(rt_step(ctx, 1, last && s == k - 1 ? 1 : 0));
             }
         }

@@ -13,6 +13,7 @@
 
 int launch_mask(rt_ctx *c);
 static int ensure_unscaled(rt_ctx *c);
+static int build_tables_on_device(rt_ctx *c, double tol);
 
 namespace {
 thread_local std::string g_create_error;
@@ -289,7 +290,7 @@ void rt_destroy(rt_ctx *c)
     dfree(c->d_w); dfree(c->d_phi); dfree(c->d_cx); dfree(c->d_cy); dfree(c->d_Nc); dfree(c->d_Wc);
     dfree(c->d_basis); dfree(c->d_dirt); dfree(c->d_alpha); dfree(c->d_t11); dfree(c->d_t12);
     dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_coefT); dfree(c->d_excl); dfree(c->d_red);
-    dfree(c->d_frame); dfree(c->d_lf); dfree(c->d_src_idx); dfree(c->d_src_I);
+    dfree(c->d_frame); dfree(c->d_lf); dfree(c->d_src_idx); dfree(c->d_src_I); dfree(c->d_metric);
     dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
     dfree(c->d_lam[0]); dfree(c->d_lam[1]); dfree(c->d_grad[0]); dfree(c->d_grad[1]); dfree(c->d_pts); dfree(c->d_mask); dfree(c->d_f64);
     dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_mom); dfree(c->d_count);
@@ -419,21 +420,29 @@ int rt_build_fourier_tables(rt_ctx *c, const double *metric, double tol)
     if (!c->has_sstencil) return rt_fail(c, RT_ERR_STATE, "rt_set_streaming_stencil must come first");
     if (!(tol > 0) || !std::isfinite(tol)) return rt_fail(c, RT_ERR_ARG, "rk45 tolerance must be positive");
     /* the block may hold NaN at a coordinate singularity inside the excision
-     * region (Kerr-Schild at r = 0); the tables are validated instead */
+     * region (Kerr-Schild at r = 0); the tables are validated instead.  The
+     * block stays on the device for rt_rebuild_fourier_tables. */
     const size_t bytes = (size_t)c->N * RT_NMETRIC * sizeof(double);
-    double *dm = nullptr;
-    TRY(dalloc(c, &dm, bytes));
-    hipError_t e = hipMemcpyAsync(dm, metric, bytes, hipMemcpyHostToDevice, c->stream);
-    int r = RT_OK;
-    if (e != hipSuccess) r = rt_hip_check(c, e, "metric upload");
-    if (r == RT_OK) r = launch_fill(c, c->d_coef, 5 * (size_t)c->nCoeff * c->N, 0.0);
-    if (r == RT_OK) r = launch_geodesic_tables(c, dm, tol);
-    if (r == RT_OK) {
-        e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) r = rt_hip_check(c, e, "geodesic tables");
-    }
-    dfree(dm);
-    if (r != RT_OK) return r;
+    TRY(dalloc(c, &c->d_metric, bytes));
+    RT_HIP(c, hipMemcpyAsync(c->d_metric, metric, bytes, hipMemcpyHostToDevice, c->stream));
+    return build_tables_on_device(c, tol);
+}
+
+int rt_rebuild_fourier_tables(rt_ctx *c, double tol)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    if (!c->d_metric) return rt_fail(c, RT_ERR_STATE, "rt_build_fourier_tables has not been called");
+    if (!(tol > 0) || !std::isfinite(tol)) return rt_fail(c, RT_ERR_ARG, "rk45 tolerance must be positive");
+    return build_tables_on_device(c, tol);
+}
+
+/* RK45 tables from the device metric block c->d_metric. */
+static int build_tables_on_device(rt_ctx *c, double tol)
+{
+    c->ck_valid = false;
+    TRY(launch_fill(c, c->d_coef, 5 * (size_t)c->nCoeff * c->N, 0.0));
+    TRY(launch_geodesic_tables(c, c->d_metric, tol));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
     c->has_tables = true;
     TRY(refresh_departure_bound(c));
     RT_HIP(c, hipStreamSynchronize(c->stream));

@@ -71,6 +71,7 @@ struct rt_ctx {
     bool has_geometry = false, has_tables = false;
     double *d_alpha = nullptr, *d_t11 = nullptr, *d_t12 = nullptr, *d_t21 = nullptr, *d_t22 = nullptr;
     double *d_coef = nullptr;     /* SoA [5 series][nCoeff][N], FP64 */
+    double *d_metric = nullptr;   /* [N][RT_NMETRIC] block of the last table build, kept for rebuilds */
     /* the same coefficients in the fused kernel's form and precision: S - 1,
      * X and Y relative to the cell centre in cells, velocity as is */
     void *d_coefT = nullptr;

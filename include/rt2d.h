@@ -124,6 +124,11 @@ int rt_set_fourier_tables(rt_ctx *ctx, const double *cS, const double *cX, const
  * metric is the per-cell block [N*RT_NMETRIC]; rk45_tol is the velocity
  * truncation tolerance (the reference hard-codes 1e-4). */
 int rt_build_fourier_tables(rt_ctx *ctx, const double *metric, double rk45_tol);
+/* The same RK45 build from the metric block of the last
+ * rt_build_fourier_tables, kept on the device: the per-step table update of
+ * Config::updateFourierHarmonics (src/Radiation.cpp:790-791) without a host
+ * transfer. */
+int rt_rebuild_fourier_tables(rt_ctx *ctx, double rk45_tol);
 /* Copies the current tables back to the host ([N*nCoeff] each). */
 int rt_get_fourier_tables(rt_ctx *ctx, double *cS, double *cX, double *cY, double *cCx,
                           double *cCy);

@@ -10,23 +10,23 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / (n if n > 0 else 1.0)
 
 
-def oracle_problem(O, grid, nDir, kind, fields, bh_r2=-1.0, tol=1e-4, tables=None, fast=False):
-    """Oracle Problem for a grid given as an oracle Grid."""
-    M = O.metric(grid, kind)
+def oracle_problem(O, grid, nDir, kind, fields, bh_r2=-1.0, tol=1e-4, tables=None, fast=False, a=0.0):
+    """Oracle Problem for a grid given as an oracle Grid (spin a for Kerr-Schild)."""
+    M = O.metric(grid, kind, 1.0, a)
     streaming = 0 if kind == 0 else 2
     if streaming == 2 and tables is None:
         tables = O.geodesic_tables(grid, M, bh_r2, tol)
     return O.Problem(grid, nDir, streaming, M, bh_r2=bh_r2, tables=tables, fields=fields, fast=fast), M, tables
 
 
-def lib_context(rt, gspec, nDir, kind, fields, precision, bh_r2=-1.0, tol=1e-4, tables=None):
+def lib_context(rt, gspec, nDir, kind, fields, precision, bh_r2=-1.0, tol=1e-4, tables=None, a=0.0):
     """Library Context built through its own C++ stencil/metric classes; tables
     either built on the GPU (tables=None) or uploaded."""
     curved = kind != 0
     ctx = rt.Context(gspec, nDir, rt.CURVED_FIXED if curved else rt.FLAT_FIXED, precision, bh_r2)
     w, phi, cx, cy, Nc, Wc = rt.build_stencil(nDir)
     ctx.set_stencil(w, phi, cx, cy, Nc, Wc)
-    block, alpha, t11, t12, t21, t22 = rt.build_metric(kind, 1.0, 0.0, gspec)
+    block, alpha, t11, t12, t21, t22 = rt.build_metric(kind, 1.0, a, gspec)
     ctx.set_geometry(alpha, t11, t12, t21, t22)
     if curved:
         ctx.set_streaming_stencil(*rt.build_stencil(5, tables=False))
