@@ -101,6 +101,12 @@ def lib():
     L.rt_batch_autotune.argtypes = [vp, i32, vp]
     L.rt_batch_groups.argtypes = [vp, C.POINTER(i32)]
     L.rt_build_stencil.argtypes = [i32, _dp, _dp, _dp, _dp, vp, vp]
+    L.rt_build_stencil_ghost.argtypes = [i32, i32, _dp, _dp, _dp, _dp, vp, vp]
+    L.rt_set_rotation.argtypes = [vp, _dp, vp]
+    L.rt_get_rotation.argtypes = [vp, vp, vp]
+    L.rt_update_rotation.argtypes = [vp]
+    L.rt_set_source_rotation.argtypes = [vp, _dp]
+    L.rt_set_moments.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.rt_build_metric.argtypes = [i32, d, d, i32, i32, d, d, d, d, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
@@ -123,6 +129,17 @@ def build_stencil(n, tables=True):
     Wc = np.zeros(40 * n) if tables else None
     _check(lib().rt_build_stencil(n, w, phi, cx, cy, _ptr(Nc), _ptr(Wc)))
     return (w, phi, cx, cy, Nc, Wc) if tables else (w, phi, cx, cy)
+
+
+def build_stencil_ghost(n_order, n_ghost):
+    """Stencil(n_order, n_ghost) with ghost directions, sorted by angle:
+    (w, phi, cx, cy, Nc, Wc); the cubic tables have 10*n_order fine angles."""
+    n = n_order + n_ghost
+    w, phi, cx, cy = (np.zeros(n) for _ in range(4))
+    Nc = np.zeros(40 * n_order, np.int32)
+    Wc = np.zeros(40 * n_order)
+    _check(lib().rt_build_stencil_ghost(n_order, n_ghost, w, phi, cx, cy, Nc.ctypes.data, Wc.ctypes.data))
+    return w, phi, cx, cy, Nc, Wc
 
 
 def build_flux_tables(n):
@@ -250,6 +267,31 @@ class Context:
         m = np.ascontiguousarray(is_source, dtype=np.uint8)
         v = np.ascontiguousarray(I, dtype=np.float64)
         self._c(lib().rt_set_sources(self.h, m.ctypes.data, v.ctypes.data))
+
+    def set_rotation(self, angle, angle_new=None):
+        """rotationAngle and rotationAngleNew ([N] each; None: 0) of an
+        adaptive context."""
+        a = np.ascontiguousarray(angle, dtype=np.float64)
+        b = None if angle_new is None else np.ascontiguousarray(angle_new, dtype=np.float64)
+        self._c(lib().rt_set_rotation(self.h, a, _ptr(b)))
+
+    def rotation(self):
+        """(rotationAngle, rotationAngleNew) as held on the device."""
+        a, b = np.zeros(self.grid.N), np.zeros(self.grid.N)
+        self._c(lib().rt_get_rotation(self.h, a.ctypes.data, b.ctypes.data))
+        return a, b
+
+    def update_rotation(self):
+        self._c(lib().rt_update_rotation(self.h))
+
+    def set_source_rotation(self, angle):
+        """Angle re-imposed on each source cell before every step ([N], NaN: none)."""
+        self._c(lib().rt_set_source_rotation(self.h, np.ascontiguousarray(angle, dtype=np.float64)))
+
+    def set_moments(self, mom):
+        """E, Fx, Fy, Pxx, Pxy, Pyy ([6][N]; a None row is left as it is)."""
+        rows = [None if m is None else np.ascontiguousarray(m, dtype=np.float64) for m in mom]
+        self._c(lib().rt_set_moments(self.h, *[_ptr(r) for r in rows]))
 
     def set_lab_frame(self, tetrad, gamma):
         """Full tetrad e[a][A] as [9][N] (00, 01, ..., 22) and gamma_ll as

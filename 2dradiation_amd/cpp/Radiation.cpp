@@ -18,9 +18,24 @@ void Radiation::Check(int status) const
 
 void Radiation::RequireSupportedStreaming() const
 {
-    if (config.streamingType != StreamingType::FlatFixed && config.streamingType != StreamingType::CurvedFixed)
-        ExitOnError("streaming type " + StreamingName(config.streamingType) +
-                    " is not part of this library (FlatFixed and CurvedFixed are)");
+    if (config.streamingType == StreamingType::GeodesicFixed)
+        ExitOnError("streaming type GeodesicFixed (a performance test of the reference) is not part of this library");
+}
+bool Radiation::IsAdaptive() const
+{
+    return config.streamingType == StreamingType::FlatAdaptive || config.streamingType == StreamingType::CurvedAdaptive;
+}
+bool Radiation::IsCurved() const
+{
+    return config.streamingType == StreamingType::CurvedFixed || config.streamingType == StreamingType::CurvedAdaptive;
+}
+void Radiation::UploadRotation()
+{
+    if (IsAdaptive()) Check(rt_set_rotation(ctx, rotationAngle.data(), rotationAngleNew.data()));
+}
+void Radiation::DownloadRotation()
+{
+    if (IsAdaptive()) Check(rt_get_rotation(ctx, rotationAngle.data(), rotationAngleNew.data()));
 }
 
 // Reference constructor: src/Radiation.cpp:3-82.  Buffers get the same sizes;
@@ -75,7 +90,7 @@ Radiation::Radiation(Metric &metric_, Stencil &stencil_, Stencil &streamingStenc
     desc.nDir = (int32_t)D;
     desc.nCoeff = (int32_t)nF;
     desc.precision = config.precision == Precision::FP32Mixed ? RT_FP32_MIXED : RT_FP64;
-    desc.streaming = config.streamingType == StreamingType::CurvedFixed ? RT_CURVED_FIXED : RT_FLAT_FIXED;
+    desc.streaming = (int32_t)config.streamingType; /* rt_streaming mirrors StreamingType */
     desc.excision_r2 = config.excisionRadius >= 0 ? config.excisionRadius * config.excisionRadius : metric.ExcisionR2();
     desc.device = config.device;
     if (rt_create(&desc, &ctx) != RT_OK) ExitOnError(std::string("rt2d: ") + rt_last_error(nullptr));
@@ -152,15 +167,22 @@ double Radiation::NormalizationOverwrite() const
 // given (InitialDataType::Intensities), or (Moments) a von Mises distribution
 // I_d = E exp(sigma n.C_d - normalization) around the inertial-frame flux
 // direction n, with sigma and the normalization from the stencil's
-// flux-to-sigma tables at |F|/E (or sigmaOverwrite).  Fixed streaming keeps
-// the rotation angle 0.
+// flux-to-sigma tables at |F|/E (or sigmaOverwrite).  Adaptive streaming
+// puts the distribution at angle 0 of the cell's stencil and turns the
+// stencil to the flux direction (rotationAngle); fixed streaming sets the
+// angle to 0.  Cells without flux keep their angle.  sourceAngle records, per
+// cell, the angle written here (NaN: none), which keepSourceNodesActive
+// re-imposes with the intensities.
 void Radiation::LoadInitialData()
 {
+    const bool adaptive = IsAdaptive();
     const double normalizationOverwrite = (0 <= sigmaOverwrite) ? NormalizationOverwrite() : 0.0;
+    sourceAngle.assign(grid.nxy, std::nan(""));
     for (size_t ij = 0; ij < grid.nxy; ij++) {
         if (!isInitialGridPoint[ij]) continue;
-        rotationAngle[ij] = 0;
         if (config.initialDataType == InitialDataType::Intensities) {
+            rotationAngle[ij] = adaptive ? initialFluxAngle_IF[ij] : 0.0;
+            sourceAngle[ij] = rotationAngle[ij];
             for (size_t d = 0; d < stencil.nDir; d++) I[Index(ij, d)] = initialI[Index(ij, d)];
             continue;
         }
@@ -169,8 +191,12 @@ void Radiation::LoadInitialData()
         const Tensor2 F(m[1], m[2]);
         const double Fn = F.EuklNorm();
         const double relativeF = Fn / E_IF;
-        Tensor2 dir(m[1] / Fn, m[2] / Fn);
+        Tensor2 dir = adaptive ? Tensor2(1, 0) : Tensor2(m[1] / Fn, m[2] / Fn);
         if (Fn < MIN_FLUX_NORM) dir = Tensor2(1, 0);
+        if (!(Fn < MIN_FLUX_NORM)) {
+            rotationAngle[ij] = adaptive ? std::fmod(MyAtan2(m[2], m[1]) + 2.0 * rtm::kPi, 2.0 * rtm::kPi) : 0.0;
+            sourceAngle[ij] = rotationAngle[ij];
+        }
         double sigma = stencil.fluxToSigmaTable.Evaluate(relativeF);
         double normalization = stencil.fluxToNormalizationTable.Evaluate(relativeF);
         if (0 <= sigmaOverwrite) {
@@ -185,7 +211,7 @@ void Radiation::LoadInitialData()
 // Reference: src/Radiation.cpp:191-237 (GPU RK45 backtrace, FP64).
 void Radiation::UpdateFourierCoefficients()
 {
-    if (config.streamingType != StreamingType::CurvedFixed) return;
+    if (!IsCurved()) return;
     std::vector<double> block(grid.nxy * RT_NMETRIC);
     metric.PackGeodesicBlock(block.data());
     Check(rt_build_fourier_tables(ctx, block.data(), config.rk45Tolerance));
@@ -199,6 +225,7 @@ void Radiation::ComputeMomentsIF()
 {
     UploadFields();
     UploadI();
+    UploadRotation();
     Check(rt_compute_moments(ctx));
     DownloadMoments();
 }
@@ -209,6 +236,7 @@ void Radiation::ComputeMomentsIF()
 void Radiation::ComputeMomentsLF()
 {
     UploadI();
+    UploadRotation();
     Check(rt_compute_moments(ctx));
     DownloadMoments();
     DownloadMomentsLF();
@@ -245,8 +273,45 @@ void Radiation::StreamCurvedFixed()
     DownloadI();
 }
 
-void Radiation::StreamFlatAdaptive() { ExitOnError("FlatAdaptive streaming is not part of this library"); }
-void Radiation::StreamCurvedAdaptive() { ExitOnError("CurvedAdaptive streaming is not part of this library"); }
+// Reference: src/Radiation.cpp:361-378.  The new angles of the interior
+// cells come from the 3x3 average of Fx, Fy; the result is rotationAngleNew.
+void Radiation::UpdateRotationMatrizes()
+{
+    if (!IsAdaptive()) ExitOnError("UpdateRotationMatrizes needs a Radiation built with adaptive streaming");
+    Check(rt_set_moments(ctx, E.data(), Fx.data(), Fy.data(), Pxx.data(), Pxy.data(), Pyy.data()));
+    UploadRotation();
+    Check(rt_update_rotation(ctx));
+    DownloadRotation();
+}
+
+// Reference: src/Radiation.cpp:415-453 and 513-575: stencils turned by
+// rotationAngleNew, neighbours read in their rotationAngle frames; I and the
+// two angle buffers are swapped afterwards, as in the reference.
+void Radiation::StreamFlatAdaptive()
+{
+    if (config.streamingType != StreamingType::FlatAdaptive)
+        ExitOnError("StreamFlatAdaptive needs a Radiation built with StreamingType::FlatAdaptive");
+    UploadFields();
+    UploadI();
+    UploadRotation();
+    Check(rt_stream(ctx));
+    std::swap(I, Inew);
+    DownloadI();
+    DownloadRotation();
+}
+void Radiation::StreamCurvedAdaptive()
+{
+    if (config.streamingType != StreamingType::CurvedAdaptive)
+        ExitOnError("StreamCurvedAdaptive needs a Radiation built with StreamingType::CurvedAdaptive");
+    if (!tablesOnDevice) UpdateFourierCoefficients();
+    UploadFields();
+    UploadI();
+    UploadRotation();
+    Check(rt_stream(ctx));
+    std::swap(I, Inew);
+    DownloadI();
+    DownloadRotation();
+}
 void Radiation::StreamGeodesicFixed() { ExitOnError("GeodesicFixed streaming is not part of this library"); }
 
 // Reference: src/Radiation.cpp:641-736.  The lambda iteration starts from the
@@ -254,9 +319,10 @@ void Radiation::StreamGeodesicFixed() { ExitOnError("GeodesicFixed streaming is 
 // afterwards I holds the collided state and E..Pyy the final moments.
 void Radiation::Collide()
 {
-    if (config.streamingType == StreamingType::CurvedFixed && !tablesOnDevice) UpdateFourierCoefficients();
+    if (IsCurved() && !tablesOnDevice) UpdateFourierCoefficients();
     UploadFields();
     UploadI();
+    UploadRotation();
     Check(rt_compute_moments(ctx));
     Check(rt_reset_iteration_stats(ctx));
     Check(rt_collide(ctx));
@@ -347,12 +413,22 @@ void Radiation::RunSimulation()
 
     UploadFields();
     UploadI();
+    if (IsAdaptive()) {
+        /* UpdateRotationMatrizes of the first step reads the moments held now */
+        UploadRotation();
+        Check(rt_set_moments(ctx, E.data(), Fx.data(), Fy.data(), Pxx.data(), Pxy.data(), Pyy.data()));
+    }
     if (config.keepSourceNodesActive) {
-        /* the source cells' initial intensities are re-imposed on the GPU
-         * before every step */
+        /* the source cells' initial intensities (and, for adaptive streaming,
+         * their angles) are re-imposed on the GPU before every step */
         std::vector<uint8_t> src(grid.nxy);
-        for (size_t ij = 0; ij < grid.nxy; ij++) src[ij] = isInitialGridPoint[ij] ? 1 : 0;
+        bool any = false;
+        for (size_t ij = 0; ij < grid.nxy; ij++) {
+            src[ij] = isInitialGridPoint[ij] ? 1 : 0;
+            any = any || isInitialGridPoint[ij];
+        }
         Check(rt_set_sources(ctx, src.data(), I.data()));
+        if (IsAdaptive() && any) Check(rt_set_source_rotation(ctx, sourceAngle.data()));
     }
     Check(rt_reset_iteration_stats(ctx));
     double currentTime = config.t0, timeSinceLastFrame = 0, computeSeconds = 0;
@@ -384,8 +460,7 @@ void Radiation::RunSimulation()
             // per-step table rebuild of the reference loop (src/Radiation.cpp:790-791),
             // from the metric block kept on the device
             for (int s = 0; s < k; s++) {
-                if (config.streamingType == StreamingType::CurvedFixed)
-                    Check(rt_rebuild_fourier_tables(ctx, config.rk45Tolerance));
+                if (IsCurved()) Check(rt_rebuild_fourier_tables(ctx, config.rk45Tolerance));
                 Check(rt_step(ctx, 1, last && s == k - 1 ? 1 : 0));
             }
         }
@@ -397,6 +472,7 @@ void Radiation::RunSimulation()
     if (timeSteps > 0 && !(config.writeData && timeSinceLastFrame == 0)) DownloadMoments();
     if (config.writeData && timeSinceLastFrame != 0) frame();
     DownloadI();
+    DownloadRotation();
 
     int32_t mx = 0;
     double sum = 0;

@@ -13,8 +13,11 @@
 // Frame output (writeData) follows the reference: lab-frame moments computed
 // on the GPU, CSV frames under OUTPUTDIR/<name>/Moments/ and Log.txt.
 //
-// Not part of this library (the methods exit with a message, like the
-// reference's ExitOnError): adaptive and geodesic streaming.
+// Streaming types: FlatFixed, CurvedFixed (fused kernel where it applies) and
+// FlatAdaptive, CurvedAdaptive (per-cell rotated stencils; rotationAngle and
+// rotationAngleNew travel to and from the GPU with the state).  Not part of
+// this library (exits with a message, like the reference's ExitOnError):
+// GeodesicFixed, the reference's performance test.
 #ifndef RT2D_RADIATION_H
 #define RT2D_RADIATION_H
 #include "Config.hh"
@@ -42,6 +45,11 @@ class Radiation {
     void WriteLog(const std::string &directoryPath, const std::string &date, int timeSteps,
                   double computeSeconds) const;
     void RequireSupportedStreaming() const;
+    bool IsAdaptive() const;
+    bool IsCurved() const;
+    void UploadRotation();
+    void DownloadRotation();
+    std::vector<double> sourceAngle; // angle LoadInitialData wrote per cell, NaN: none
     Tensor3 InitialDataLFtoIF(size_t ij) const;
     double NormalizationOverwrite() const;
 
@@ -88,6 +96,7 @@ class Radiation {
     void UpdateFourierCoefficients();
     void ComputeMomentsIF();
     void ComputeMomentsLF();
+    void UpdateRotationMatrizes();
 
     void StreamFlatFixed();
     void StreamFlatAdaptive();

@@ -45,27 +45,105 @@ double InterpolationGrid::d(double phi) const
     return std::clamp(nGrid * phi / (2.0 * rtm::kPi), 1e-8, nGrid - 1e-8);
 }
 
-// Stencil (reference src/Stencil.cpp:177-297) without ghost directions:
-// uniform weights 1/n at angles 2*pi*(d+1/2)/n, x-cosines through MyCos.
+namespace {
+// Ghost directions of Stencil(nOrder, nGhost) (reference src/Stencil.cpp:202-266).
+// Buckets are the gaps between neighbouring Fourier directions that both lie
+// within deltaPhi of pi (or, if there are none, the gap around pi); nGhost
+// samples of the arccos distribution over the buckets' span decide how many
+// ghosts each bucket gets; they are spread evenly inside their bucket and the
+// whole set is turned by pi, so the ghosts sit around the forward direction
+// phi = 0 of the rotated stencil.  Returns false if the counts miss nGhost.
+bool GhostAngles(size_t nOrder, int nGhost, std::vector<double> &out)
+{
+    const double twoPi = 2.0 * rtm::kPi;
+    auto circ = [&](double a, double b) {
+        const double delta = std::abs(a - b);
+        return std::min(delta, twoPi - delta);
+    };
+    std::vector<double> lo, hi;
+    for (size_t d = 0; d < nOrder; d++) {
+        const double p0 = 2.0 * rtm::kPi * (d + 0.5) / nOrder;
+        const double p1 = 2.0 * rtm::kPi * ((d + 1) % nOrder + 0.5) / nOrder;
+        if (circ(p0, rtm::kPi) <= Stencil::deltaPhi && circ(p1, rtm::kPi) <= Stencil::deltaPhi) {
+            lo.push_back(p0);
+            hi.push_back(p1);
+        }
+    }
+    if (lo.empty()) {
+        const double first = 2.0 * rtm::kPi * 0.5 / nOrder;
+        lo.push_back(rtm::kPi - first);
+        hi.push_back(rtm::kPi + first);
+    }
+    const double from = lo.front(), to = hi.back();
+    std::vector<int> count(lo.size(), 0);
+    for (int g = 0; g < nGhost; g++) {
+        const double q = MyAcos(1.0 - 2.0 * ((g + 0.5) / nGhost)) / rtm::kPi;
+        const double angle = from + (to - from) * q;
+        for (size_t b = 0; b < lo.size(); b++)
+            if (circ(angle, (lo[b] + hi[b]) / 2.0) < (hi[b] - lo[b]) / 2.0) count[b]++;
+    }
+    out.clear();
+    for (size_t b = 0; b < lo.size(); b++)
+        for (int k = 0; k < count[b]; k++) {
+            const double a = lo[b] + (hi[b] - lo[b]) * (k + 0.5) / count[b] + rtm::kPi;
+            out.push_back(std::fmod(a + twoPi, twoPi));
+        }
+    return (int)out.size() == nGhost;
+}
+} // namespace
+
+bool Stencil::GhostLayoutValid(size_t nOrder, int nGhost, std::string *why)
+{
+    if (nGhost < 0) {
+        if (why) *why = "nGhost must be >= 0";
+        return false;
+    }
+    if (nGhost == 0) return true;
+    if (nOrder % 2 != 0) {
+        if (why) *why = "Stencil odd nOrder not allowed when using nGhost>0.";
+        return false;
+    }
+    std::vector<double> g;
+    if (!GhostAngles(nOrder, nGhost, g)) {
+        if (why) *why = "ghost directions of Stencil(" + std::to_string(nOrder) + "," + std::to_string(nGhost) +
+                        ") do not fill their buckets";
+        return false;
+    }
+    return true;
+}
+
+// Stencil (reference src/Stencil.cpp:177-297, 299-332): nOrder Fourier
+// directions with weights 1/nOrder at angles 2*pi*(d+1/2)/nOrder (x-cosines
+// through MyCos), nGhost weight-0 ghost directions, all sorted by angle.
 Stencil::Stencil(size_t nOrder_, int nGhost_, bool isIntensityStencil)
 {
-    if (nGhost_ != 0)
-        ExitOnError("Stencil with nGhost>0 (adaptive streaming) is not supported by this library.");
+    if (!isIntensityStencil && nGhost_ > 0) ExitOnError("Stencil with nGhost>0 is only allowed for intensity stencils.");
     if (nOrder_ < 4 && isIntensityStencil) ExitOnError("Intensity stencil needs at least 4 directions.");
+    std::string why;
+    if (!GhostLayoutValid(nOrder_, nGhost_, &why)) ExitOnError(why);
     name = "Stencil" + std::to_string(nOrder_) + "." + std::to_string(nGhost_);
-    nDir = nOrder_;
-    nGhost = 0;
+    nGhost = (size_t)nGhost_;
     nOrder = nOrder_;
+    nDir = nOrder + nGhost;
     nCoefficients = nOrder_;
+    std::vector<std::array<double, 4>> dirs; // (phi, w, cx, cy)
+    for (size_t d = 0; d < nOrder; d++) {
+        const double p = 2.0 * rtm::kPi * (d + 0.5) / nOrder;
+        dirs.push_back({p, 1.0 / nOrder, MyCos(p), MySin(p)});
+    }
+    std::vector<double> ghosts;
+    if (nGhost > 0) GhostAngles(nOrder, nGhost_, ghosts);
+    for (double p : ghosts) dirs.push_back({p, 0.0, MyCos(p), MySin(p)});
+    std::stable_sort(dirs.begin(), dirs.end(), [](const auto &a, const auto &b) { return a[0] < b[0]; });
     w.resize(nDir);
     phi.resize(nDir);
     cx.resize(nDir);
     cy.resize(nDir);
     for (size_t d = 0; d < nDir; d++) {
-        w[d] = 1.0 / nDir;
-        phi[d] = 2.0 * rtm::kPi * (d + 0.5) / nDir;
-        cx[d] = MyCos(phi[d]);
-        cy[d] = MySin(phi[d]);
+        phi[d] = dirs[d][0];
+        w[d] = dirs[d][1];
+        cx[d] = dirs[d][2];
+        cy[d] = dirs[d][3];
     }
     if (isIntensityStencil) {
         interpolationGrid = InterpolationGrid(10 * nOrder, *this);

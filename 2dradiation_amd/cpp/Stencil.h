@@ -1,7 +1,7 @@
 // Velocity stencil, its cubic angular interpolation grid and the
 // flux-to-sigma lookup tables of the von Mises initial data (reference:
-// src/Stencil.h, src/Stencil.cpp:4-162, 177-297, 378-455).  Ghost directions
-// (adaptive streaming) are not part of this library: nGhost must be 0.
+// src/Stencil.h, src/Stencil.cpp:4-162, 177-297, 378-455).  Intensity
+// stencils may carry nGhost weight-0 ghost directions for adaptive streaming.
 #ifndef RT2D_STENCIL_H
 #define RT2D_STENCIL_H
 #include <array>
@@ -41,6 +41,9 @@ struct Stencil {
 
   public:
     Stencil(size_t nOrder, int nGhost = 0, bool isIntensityStencil = true);
+    // Whether Stencil(nOrder, nGhost) has a well-defined ghost layout (even
+    // nOrder, ghost counts per bucket adding up to nGhost); why gets the reason.
+    static bool GhostLayoutValid(size_t nOrder, int nGhost, std::string *why = nullptr);
     double W(size_t d) const { return w[d]; }
     double Phi(size_t d) const { return phi[d]; }
     double Cx(size_t d) const { return cx[d]; }

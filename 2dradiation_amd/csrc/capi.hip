@@ -221,9 +221,10 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
     if (d.nDir < 4 || d.nDir > 4096) return rt_fail(nullptr, RT_ERR_ARG, "nDir must be in [4, 4096]");
     if (d.nCoeff < 1 || d.nCoeff > 8) return rt_fail(nullptr, RT_ERR_ARG, "nCoeff must be in [1, 8]");
     if (d.precision != RT_FP64 && d.precision != RT_FP32_MIXED) return rt_fail(nullptr, RT_ERR_ARG, "unknown precision");
-    if (d.streaming == RT_FLAT_ADAPTIVE || d.streaming == RT_CURVED_ADAPTIVE || d.streaming == RT_GEODESIC_FIXED)
-        return rt_fail(nullptr, RT_ERR_UNSUPPORTED, "only FlatFixed and CurvedFixed streaming are implemented");
-    if (d.streaming != RT_FLAT_FIXED && d.streaming != RT_CURVED_FIXED) return rt_fail(nullptr, RT_ERR_ARG, "Invalid StreamingType");
+    if (d.streaming == RT_GEODESIC_FIXED)
+        return rt_fail(nullptr, RT_ERR_UNSUPPORTED, "GeodesicFixed streaming (a performance test of the reference) is not implemented");
+    if (d.streaming != RT_FLAT_FIXED && d.streaming != RT_CURVED_FIXED && !rt_is_adaptive(d))
+        return rt_fail(nullptr, RT_ERR_ARG, "Invalid StreamingType");
     if ((long long)d.nx * d.ny * d.nDir > (1LL << 33)) return rt_fail(nullptr, RT_ERR_ARG, "problem too large");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
@@ -260,10 +261,19 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
         (r = dalloc(c, &c->d_cx, D * sizeof(double))) || (r = dalloc(c, &c->d_cy, D * sizeof(double))) ||
         (r = dalloc(c, &c->d_basis, D * d.nCoeff * sizeof(double))))
         return fail(r);
-    if (d.streaming == RT_CURVED_FIXED && ((r = dalloc(c, &c->d_coef, 5 * (size_t)d.nCoeff * N * sizeof(double))) ||
-                                           (r = dalloc(c, &c->d_coefT, 5 * (size_t)d.nCoeff * N * tsize(c))) ||
+    if (rt_is_curved(d) && (r = dalloc(c, &c->d_coef, 5 * (size_t)d.nCoeff * N * sizeof(double)))) return fail(r);
+    if (d.streaming == RT_CURVED_FIXED && ((r = dalloc(c, &c->d_coefT, 5 * (size_t)d.nCoeff * N * tsize(c))) ||
                                            (r = dalloc(c, &c->d_excl, N * sizeof(unsigned long long)))))
         return fail(r);
+    if (rt_is_adaptive(d)) {
+        if ((r = dalloc(c, &c->d_rot[0], N * sizeof(double))) || (r = dalloc(c, &c->d_rot[1], N * sizeof(double))) ||
+            (r = dalloc(c, &c->d_rtrig, 4 * N * sizeof(double))))
+            return fail(r);
+        /* rotationAngle = rotationAngleNew = 0 (src/Radiation.cpp:55) */
+        if (hipMemsetAsync(c->d_rot[0], 0, N * sizeof(double), c->stream) != hipSuccess ||
+            hipMemsetAsync(c->d_rot[1], 0, N * sizeof(double), c->stream) != hipSuccess)
+            return fail(rt_fail(c, RT_ERR_HIP, "initialisation failed"));
+    }
     if ((r = dalloc(c, &c->d_red, sizeof(unsigned long long)))) return fail(r);
     /* zeroed state; flat geometry until rt_set_geometry (alpha = 1, tetrad = I) */
     if (hipMemsetAsync(c->d_I[0], 0, N * D * tsize(c), c->stream) != hipSuccess ||
@@ -291,6 +301,7 @@ void rt_destroy(rt_ctx *c)
     dfree(c->d_basis); dfree(c->d_dirt); dfree(c->d_alpha); dfree(c->d_t11); dfree(c->d_t12);
     dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_coefT); dfree(c->d_excl); dfree(c->d_red);
     dfree(c->d_frame); dfree(c->d_lf); dfree(c->d_src_idx); dfree(c->d_src_I); dfree(c->d_metric);
+    dfree(c->d_rot[0]); dfree(c->d_rot[1]); dfree(c->d_rtrig); dfree(c->d_src_rot);
     dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
     dfree(c->d_lam[0]); dfree(c->d_lam[1]); dfree(c->d_grad[0]); dfree(c->d_grad[1]); dfree(c->d_pts); dfree(c->d_mask); dfree(c->d_f64);
     dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_mom); dfree(c->d_count);
@@ -316,8 +327,8 @@ int rt_set_stencil(rt_ctx *c, const double *w, const double *phi, const double *
     const int D = c->D;
     if (!all_finite(w, D) || !all_finite(phi, D) || !all_finite(cx, D) || !all_finite(cy, D))
         return rt_fail(c, RT_ERR_NONFINITE, "stencil arrays contain NaN/Inf");
-    if (c->desc.streaming == RT_CURVED_FIXED) {
-        if (nGrid < 2 || !Nc || !Wc) return rt_fail(c, RT_ERR_ARG, "curved streaming needs the cubic interpolation tables");
+    if (rt_is_curved(c->desc) || rt_is_adaptive(c->desc)) {
+        if (nGrid < 2 || !Nc || !Wc) return rt_fail(c, RT_ERR_ARG, "curved and adaptive streaming need the cubic interpolation tables");
         for (long k = 0; k < 4L * nGrid; k++)
             if (Nc[k] < 0 || Nc[k] >= D) return rt_fail(c, RT_ERR_ARG, "interpolation table index out of range");
         if (!all_finite(Wc, 4L * nGrid)) return rt_fail(c, RT_ERR_NONFINITE, "interpolation weights contain NaN/Inf");
@@ -395,7 +406,7 @@ int rt_set_geometry(rt_ctx *c, const double *alpha, const double *t11, const dou
 int rt_set_fourier_tables(rt_ctx *c, const double *cS, const double *cX, const double *cY, const double *cCx, const double *cCy)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
-    if (c->desc.streaming != RT_CURVED_FIXED) return rt_fail(c, RT_ERR_STATE, "Fourier tables are only used by CurvedFixed streaming");
+    if (!rt_is_curved(c->desc)) return rt_fail(c, RT_ERR_STATE, "Fourier tables are only used by curved streaming");
     NEED(c, cS); NEED(c, cX); NEED(c, cY); NEED(c, cCx); NEED(c, cCy);
     const size_t N = c->N, F = c->nCoeff;
     const double *src[5] = {cS, cX, cY, cCx, cCy};
@@ -415,7 +426,7 @@ int rt_set_fourier_tables(rt_ctx *c, const double *cS, const double *cX, const d
 int rt_build_fourier_tables(rt_ctx *c, const double *metric, double tol)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
-    if (c->desc.streaming != RT_CURVED_FIXED) return rt_fail(c, RT_ERR_STATE, "Fourier tables are only used by CurvedFixed streaming");
+    if (!rt_is_curved(c->desc)) return rt_fail(c, RT_ERR_STATE, "Fourier tables are only used by curved streaming");
     NEED(c, metric);
     if (!c->has_sstencil) return rt_fail(c, RT_ERR_STATE, "rt_set_streaming_stencil must come first");
     if (!(tol > 0) || !std::isfinite(tol)) return rt_fail(c, RT_ERR_ARG, "rk45 tolerance must be positive");
@@ -510,8 +521,12 @@ int rt_upload_I(rt_ctx *c, const double *I)
     if (e != hipSuccess) r = rt_hip_check(c, e, "upload I");
     if (r == RT_OK) r = launch_transpose_in(c, tmp, c->d_I[c->cur]);
     if (r == RT_OK) {
-        e = hipMemcpyAsync(c->d_I[1 - c->cur], c->d_I[c->cur], n * tsize(c), hipMemcpyDeviceToDevice, c->stream);
-        if (e != hipSuccess) r = rt_hip_check(c, e, "copy I");
+        /* the scratch buffer starts as the reference's Inew: zero, so that the
+         * halo of every streamed state is 0 (its moments feed the rotation
+         * update of adaptive streaming); the fused path copies the fixed cells
+         * itself before it steps */
+        e = hipMemsetAsync(c->d_I[1 - c->cur], 0, n * tsize(c), c->stream);
+        if (e != hipSuccess) r = rt_hip_check(c, e, "clear scratch I");
     }
     if (r == RT_OK) {
         e = hipStreamSynchronize(c->stream);
@@ -603,10 +618,17 @@ int rt_set_sources(rt_ctx *c, const uint8_t *isSource, const double *I)
     if (n > c->src_cap) {
         dfree(c->d_src_idx);
         dfree(c->d_src_I);
+        dfree(c->d_src_rot);
+        c->d_src_idx = nullptr;
+        c->d_src_I = nullptr;
+        c->d_src_rot = nullptr;
+        c->src_cap = 0;
         TRY(dalloc(c, &c->d_src_idx, (size_t)n * sizeof(int32_t)));
         TRY(dalloc(c, &c->d_src_I, soa.size() * sizeof(double)));
+        if (rt_is_adaptive(c->desc)) TRY(dalloc(c, &c->d_src_rot, (size_t)n * sizeof(double)));
         c->src_cap = n;
     }
+    c->src_rot_set = false;
     if (n > 0) {
         RT_HIP(c, hipMemcpyAsync(c->d_src_idx, idx.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
         RT_HIP(c, hipMemcpyAsync(c->d_src_I, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -686,10 +708,9 @@ static int check_ready(rt_ctx *c)
 {
     if (!c->has_stencil) return rt_fail(c, RT_ERR_STATE, "rt_set_stencil has not been called");
     if (!c->has_fields) return rt_fail(c, RT_ERR_STATE, "rt_set_fields has not been called");
-    if (c->desc.streaming == RT_CURVED_FIXED) {
-        if (!c->has_tables) return rt_fail(c, RT_ERR_STATE, "curved streaming needs Fourier tables");
-        if (!c->d_Nc) return rt_fail(c, RT_ERR_STATE, "curved streaming needs the cubic interpolation tables");
-    }
+    if (rt_is_curved(c->desc) && !c->has_tables) return rt_fail(c, RT_ERR_STATE, "curved streaming needs Fourier tables");
+    if ((rt_is_curved(c->desc) || rt_is_adaptive(c->desc)) && !c->d_Nc)
+        return rt_fail(c, RT_ERR_STATE, "curved and adaptive streaming need the cubic interpolation tables");
     return RT_OK;
 }
 
@@ -715,6 +736,21 @@ int rt_step(rt_ctx *c, int32_t nSteps, int32_t want_moments)
             TRY(launch_fused_step(c, c->d_I[c->cur], c->d_I[1 - c->cur], want_moments && n == nSteps - 1));
             c->cur = 1 - c->cur;
         }
+    } else if (rt_is_adaptive(c->desc)) {
+        /* RunSimulation (src/Radiation.cpp:784-819): LoadInitialData for the
+         * source cells, UpdateRotationMatrizes, adaptive stream (swapping
+         * I and the angle buffers), ComputeMomentsIF, Collide */
+        for (int n = 0; n < nSteps; n++) {
+            TRY(launch_restore_sources(c, c->d_I[c->cur], false));
+            TRY(launch_restore_source_rot(c));
+            TRY(launch_rotation(c, true));
+            TRY(launch_stream_adaptive(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+            c->cur = 1 - c->cur;
+            c->rcur = 1 - c->rcur;
+            TRY(launch_ref_moments(c, c->d_I[c->cur]));
+            TRY(launch_ref_collide(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+            c->cur = 1 - c->cur;
+        }
     } else {
         TRY(ensure_unscaled(c));
         for (int n = 0; n < nSteps; n++) {
@@ -736,8 +772,89 @@ int rt_stream(rt_ctx *c)
     c->ck_valid = false;
     TRY(check_ready(c));
     TRY(ensure_unscaled(c));
-    TRY(launch_ref_stream(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+    if (rt_is_adaptive(c->desc)) {
+        TRY(launch_rotation(c, false));
+        TRY(launch_stream_adaptive(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+        c->rcur = 1 - c->rcur;
+    } else {
+        TRY(launch_ref_stream(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+    }
     c->cur = 1 - c->cur;
+    return RT_OK;
+}
+
+int rt_update_rotation(rt_ctx *c)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    if (!rt_is_adaptive(c->desc)) return rt_fail(c, RT_ERR_STATE, "rotation angles belong to adaptive streaming");
+    return launch_rotation(c, true);
+}
+
+/* angles the reference's MySin/MyCos accept (src/Utility.hh:272-359) */
+static bool angles_ok(const double *a, size_t n)
+{
+    for (size_t k = 0; k < n; k++)
+        if (!(a[k] >= -rtm::kPi2 && a[k] <= 5.0 * rtm::kPi2)) return false;
+    return true;
+}
+
+int rt_set_rotation(rt_ctx *c, const double *angle, const double *angleNew)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    if (!rt_is_adaptive(c->desc)) return rt_fail(c, RT_ERR_STATE, "rotation angles belong to adaptive streaming");
+    NEED(c, angle);
+    const size_t N = c->N;
+    if (!angles_ok(angle, N) || (angleNew && !angles_ok(angleNew, N)))
+        return rt_fail(c, RT_ERR_ARG, "rotation angles must lie in [-pi/2, 5pi/2]");
+    RT_HIP(c, hipMemcpyAsync(c->d_rot[c->rcur], angle, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (angleNew) RT_HIP(c, hipMemcpyAsync(c->d_rot[1 - c->rcur], angleNew, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    else RT_HIP(c, hipMemsetAsync(c->d_rot[1 - c->rcur], 0, N * sizeof(double), c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_get_rotation(rt_ctx *c, double *angle, double *angleNew)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    if (!rt_is_adaptive(c->desc)) return rt_fail(c, RT_ERR_STATE, "rotation angles belong to adaptive streaming");
+    const size_t N = c->N;
+    if (angle) RT_HIP(c, hipMemcpyAsync(angle, c->d_rot[c->rcur], N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (angleNew) RT_HIP(c, hipMemcpyAsync(angleNew, c->d_rot[1 - c->rcur], N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_set_source_rotation(rt_ctx *c, const double *angle)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    if (!rt_is_adaptive(c->desc)) return rt_fail(c, RT_ERR_STATE, "rotation angles belong to adaptive streaming");
+    NEED(c, angle);
+    if (c->nsrc == 0) return rt_fail(c, RT_ERR_STATE, "rt_set_sources has not named any source cell");
+    std::vector<int32_t> idx(c->nsrc);
+    RT_HIP(c, hipMemcpyAsync(idx.data(), c->d_src_idx, idx.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<double> v(c->nsrc);
+    for (int k = 0; k < c->nsrc; k++) {
+        v[k] = angle[idx[k]];
+        if (v[k] == v[k] && !angles_ok(&v[k], 1)) return rt_fail(c, RT_ERR_ARG, "rotation angles must lie in [-pi/2, 5pi/2]");
+    }
+    RT_HIP(c, hipMemcpyAsync(c->d_src_rot, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    c->src_rot_set = true;
+    return RT_OK;
+}
+
+int rt_set_moments(rt_ctx *c, const double *E, const double *Fx, const double *Fy, const double *Pxx, const double *Pxy,
+                   const double *Pyy)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    const double *src[6] = {E, Fx, Fy, Pxx, Pxy, Pyy};
+    for (int q = 0; q < 6; q++) {
+        if (!src[q]) continue;
+        if (!all_finite(src[q], c->N)) return rt_fail(c, RT_ERR_NONFINITE, "moments contain NaN/Inf");
+        RT_HIP(c, hipMemcpyAsync(c->d_mom + (size_t)q * c->N, src[q], c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    RT_HIP(c, hipStreamSynchronize(c->stream));
     return RT_OK;
 }
 int rt_compute_moments(rt_ctx *c)
@@ -940,10 +1057,19 @@ int rt_build_flux_tables(int32_t nOrder, int32_t capacity, double *relativeFlux,
 
 int rt_build_stencil(int32_t nOrder, double *w, double *phi, double *cx, double *cy, int32_t *Nc, double *Wc)
 {
-    if (nOrder < 1 || nOrder > 4096 || !w || !phi || !cx || !cy) return rt_fail(nullptr, RT_ERR_ARG, "rt_build_stencil: bad arguments");
+    return rt_build_stencil_ghost(nOrder, 0, w, phi, cx, cy, Nc, Wc);
+}
+
+int rt_build_stencil_ghost(int32_t nOrder, int32_t nGhost, double *w, double *phi, double *cx, double *cy, int32_t *Nc,
+                           double *Wc)
+{
+    if (nOrder < 1 || nOrder > 4096 || nGhost < 0 || nGhost > 4096 || !w || !phi || !cx || !cy)
+        return rt_fail(nullptr, RT_ERR_ARG, "rt_build_stencil: bad arguments");
     if ((Nc || Wc) && nOrder < 4) return rt_fail(nullptr, RT_ERR_ARG, "rt_build_stencil: interpolation tables need nOrder >= 4");
-    Stencil s(nOrder, 0, Nc || Wc);
-    for (int d = 0; d < nOrder; d++) {
+    std::string why;
+    if (!Stencil::GhostLayoutValid(nOrder, nGhost, &why)) return rt_fail(nullptr, RT_ERR_ARG, "rt_build_stencil: " + why);
+    Stencil s(nOrder, nGhost, Nc || Wc);
+    for (int d = 0; d < nOrder + nGhost; d++) {
         w[d] = s.W(d);
         phi[d] = s.Phi(d);
         cx[d] = s.Cx(d);

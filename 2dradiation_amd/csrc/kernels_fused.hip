@@ -281,6 +281,7 @@ bool fused_supported(const rt_ctx *c)
     const int D = c->D;
     if (!(D == 16 || D == 20 || D == 24 || D == 32 || D == 48 || D == 64)) return false;
     const rt_desc &d = c->desc;
+    if (rt_is_adaptive(d)) return false;
     if (d.streaming == RT_CURVED_FIXED) {
         if (c->nCoeff != 5 || !c->fused_tables_ok) return false;
         if (!(c->max_disp < 0.999)) return false;

@@ -164,7 +164,7 @@ __device__ inline double ref_intensity_at(const T *__restrict__ I, long N, long 
     double angle = rtm::my_phi(vx, vy);
     double nG = (double)nGrid;
     double dd = nG * angle / (2.0 * rtm::kPi);
-    if (dd < 1e-8) dd = 1e-8;
+    if (!(dd >= 1e-8)) dd = 1e-8; /* NaN angle: keep the table reads in range */
     if (dd > nG - 1e-8) dd = nG - 1e-8;
     long d0 = (long)floor(dd) % nGrid;
     long d1 = (d0 + 1) % nGrid;
@@ -230,17 +230,25 @@ __global__ void k_ref_stream_curved(const T *__restrict__ I, T *__restrict__ Ine
     *out = (T)((sh * (sh * sh)) * rtm::bilinear<double>(iT - i0, jT - j0, f00, f01, f10, f11));
 }
 
-/* ---- ComputeMomentsIF (src/Radiation.cpp:239-269), every stored cell ---- */
+/* ---- ComputeMomentsIF (src/Radiation.cpp:239-269), every stored cell ----
+ * rot (adaptive streaming, else null): directions RotationMatrix(rot[ij]) C(d)
+ * (:257; src/DataTypes.hh:630-635) */
 template <typename T>
 __global__ void k_ref_moments(const T *__restrict__ I, double *__restrict__ mom, const double *__restrict__ w,
-                              const double *__restrict__ cx, const double *__restrict__ cy, int N, int D)
+                              const double *__restrict__ cx, const double *__restrict__ cy, int N, int D,
+                              const double *__restrict__ rot)
 {
     int ij = blockIdx.x * blockDim.x + threadIdx.x;
     if (ij >= N) return;
     double e = 0, fx = 0, fy = 0, pxx = 0, pxy = 0, pyy = 0;
+    const double rs = rot ? rtm::my_sin(rot[ij]) : 0.0, rc = rot ? rtm::my_cos(rot[ij]) : 1.0;
     for (int d = 0; d < D; d++) {
         if (w[d] == 0.0) continue;
         double n1 = cx[d], n2 = cy[d];
+        if (rot) {
+            n1 = cx[d] * rc - cy[d] * rs;
+            n2 = cx[d] * rs + cy[d] * rc;
+        }
         double c = w[d] * (double)I[(long)d * N + ij];
         e += c;
         fx += c * n1;
@@ -266,7 +274,7 @@ __global__ void k_ref_collide(const T *__restrict__ I, T *__restrict__ Inew, dou
                               const double *__restrict__ t22, const uint8_t *__restrict__ mask,
                               const double *__restrict__ w, const double *__restrict__ cx,
                               const double *__restrict__ cy, int N, int D, double dt,
-                              unsigned long long *stats)
+                              unsigned long long *stats, const double *__restrict__ rot)
 {
     int ij = blockIdx.x * blockDim.x + threadIdx.x;
     if (ij >= N) return;
@@ -283,6 +291,8 @@ __global__ void k_ref_collide(const T *__restrict__ I, T *__restrict__ Inew, dou
     double al = alpha[ij];
     double uu = u1 * u1 + u2 * u2;
     double lorentz = 1.0 / sqrt(1.0 - uu);
+    /* nIF = RotationMatrix(rotationAngle[ij]) * C(d) (:691) */
+    const double rs = rot ? rtm::my_sin(rot[ij]) : 0.0, rc = rot ? rtm::my_cos(rot[ij]) : 1.0;
     int cycles = 0;
     double diff = 1;
     while (fabs(diff) > 1e-4 && (cycles < 100)) {
@@ -303,6 +313,10 @@ __global__ void k_ref_collide(const T *__restrict__ I, T *__restrict__ Inew, dou
         for (int d = 0; d < D; d++) {
             long idx = (long)d * N + ij;
             double n1 = cx[d], n2 = cy[d];
+            if (rot) {
+                n1 = cx[d] * rc - cy[d] * rs;
+                n2 = cx[d] * rs + cy[d] * rc;
+            }
             double un = u1 * n1 + u2 * n2;
             double A = lorentz * (1.0 - un);
             double fac = (1 - (lorentz * un) / (lorentz + 1)) * lorentz;
@@ -706,6 +720,7 @@ int launch_prepare_cells(rt_ctx *c)
 int launch_ref_stream(rt_ctx *c, const void *in, void *out)
 {
     const rt_desc &d = c->desc;
+    if (rt_is_adaptive(d)) return launch_stream_adaptive(c, in, out);
     size_t n = (size_t)(d.nx - 2) * (d.ny - 2) * c->D;
     if (d.streaming == RT_FLAT_FIXED) {
         if (c->fp32)
@@ -812,12 +827,17 @@ int launch_moments_lf(rt_ctx *c)
     return RT_OK;
 }
 
+/* current rotation angles of an adaptive context (null: fixed streaming) */
+static const double *rot_or_null(const rt_ctx *c) { return rt_is_adaptive(c->desc) ? c->d_rot[c->rcur] : nullptr; }
+
 int launch_ref_moments(rt_ctx *c, const void *in)
 {
     if (c->fp32)
-        k_ref_moments<float><<<nblk(c->N, 256), 256, 0, c->stream>>>((const float *)in, c->d_mom, c->d_w, c->d_cx, c->d_cy, c->N, c->D);
+        k_ref_moments<float><<<nblk(c->N, 256), 256, 0, c->stream>>>((const float *)in, c->d_mom, c->d_w, c->d_cx, c->d_cy, c->N, c->D,
+                                                                     rot_or_null(c));
     else
-        k_ref_moments<double><<<nblk(c->N, 256), 256, 0, c->stream>>>((const double *)in, c->d_mom, c->d_w, c->d_cx, c->d_cy, c->N, c->D);
+        k_ref_moments<double><<<nblk(c->N, 256), 256, 0, c->stream>>>((const double *)in, c->d_mom, c->d_w, c->d_cx, c->d_cy, c->N, c->D,
+                                                                      rot_or_null(c));
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }
@@ -826,11 +846,11 @@ int launch_ref_collide(rt_ctx *c, const void *in, void *out)
     if (c->fp32)
         k_ref_collide<float><<<nblk(c->N, 128), 128, 0, c->stream>>>(
             (const float *)in, (float *)out, c->d_mom, c->d_count, c->d_f64, c->d_alpha, c->d_t11, c->d_t12, c->d_t21, c->d_t22,
-            c->d_mask, c->d_w, c->d_cx, c->d_cy, c->N, c->D, c->desc.dt, c->d_stats);
+            c->d_mask, c->d_w, c->d_cx, c->d_cy, c->N, c->D, c->desc.dt, c->d_stats, rot_or_null(c));
     else
         k_ref_collide<double><<<nblk(c->N, 128), 128, 0, c->stream>>>(
             (const double *)in, (double *)out, c->d_mom, c->d_count, c->d_f64, c->d_alpha, c->d_t11, c->d_t12, c->d_t21,
-            c->d_t22, c->d_mask, c->d_w, c->d_cx, c->d_cy, c->N, c->D, c->desc.dt, c->d_stats);
+            c->d_t22, c->d_mask, c->d_w, c->d_cx, c->d_cy, c->N, c->D, c->desc.dt, c->d_stats, rot_or_null(c));
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }

@@ -103,6 +103,16 @@ struct rt_ctx {
     int32_t *d_src_idx = nullptr;
     double *d_src_I = nullptr;
     int nsrc = 0, src_cap = 0;
+    /* adaptive streaming: rotationAngle is d_rot[rcur], rotationAngleNew
+     * d_rot[1 - rcur] (they swap with every stream, as in the reference);
+     * d_rtrig [4][N] holds sin, cos of the new angle and sin, cos of the
+     * inverse of the current angle of every cell for the next stream.
+     * d_src_rot [nsrc]: angle re-imposed on each source cell (NaN: none). */
+    double *d_rot[2] = {nullptr, nullptr};
+    int rcur = 0;
+    double *d_rtrig = nullptr;
+    double *d_src_rot = nullptr;
+    bool src_rot_set = false;
     int32_t *d_count = nullptr;   /* [N] per-method collide cycle counts */
     unsigned long long *d_stats = nullptr; /* [2] sum, max */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -147,6 +157,13 @@ int launch_fused_step_rec(rt_ctx *c, const void *in, void *out, const RecordPtrs
 int launch_scale_state(rt_ctx *c, void *buf, bool inverse);
 int launch_prepare_cells(rt_ctx *c);
 int launch_ref_stream(rt_ctx *c, const void *in, void *out);
+/* adaptive streaming (kernels_adaptive.hip): rotation update (or only the
+ * sines and cosines), the rotated stream, source-cell angles */
+int launch_rotation(rt_ctx *c, bool update);
+int launch_stream_adaptive(rt_ctx *c, const void *in, void *out);
+int launch_restore_source_rot(rt_ctx *c);
+inline bool rt_is_adaptive(const rt_desc &d) { return d.streaming == RT_FLAT_ADAPTIVE || d.streaming == RT_CURVED_ADAPTIVE; }
+inline bool rt_is_curved(const rt_desc &d) { return d.streaming == RT_CURVED_FIXED || d.streaming == RT_CURVED_ADAPTIVE; }
 int launch_ref_moments(rt_ctx *c, const void *in);
 int launch_ref_collide(rt_ctx *c, const void *in, void *out);
 int launch_fused_step(rt_ctx *c, const void *in, void *out, bool want_moments);

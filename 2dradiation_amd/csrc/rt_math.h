@@ -94,7 +94,7 @@ RT_HD inline double my_cos(double xd)
 
 /* Fourier::Basis (src/FourierHarmonics.cpp:5-15): 1, sqrt2 cos(phi),
  * sqrt2 sin(phi), sqrt2 cos(2phi), sqrt2 sin(2phi), ... */
-inline double fourier_basis(int k, double phi)
+RT_HD inline double fourier_basis(int k, double phi)
 {
     if (k == 0) return 1.0;
     else if (k % 2 != 0) return kSqrt2 * cos(phi * (k + 1.0) / 2.0);

@@ -49,9 +49,9 @@ enum rt_precision { RT_FP64 = 0, RT_FP32_MIXED = 1 };
 /* Values mirror the reference's StreamingType (src/Config.hh:6-13). */
 enum rt_streaming {
     RT_FLAT_FIXED = 0,
-    RT_FLAT_ADAPTIVE = 1,   /* not implemented: RT_ERR_UNSUPPORTED */
+    RT_FLAT_ADAPTIVE = 1,
     RT_CURVED_FIXED = 2,
-    RT_CURVED_ADAPTIVE = 3, /* not implemented: RT_ERR_UNSUPPORTED */
+    RT_CURVED_ADAPTIVE = 3,
     RT_GEODESIC_FIXED = 4   /* not implemented: RT_ERR_UNSUPPORTED */
 };
 
@@ -177,18 +177,48 @@ int rt_reset_iteration_stats(rt_ctx *ctx);
 int rt_get_iteration_counts(rt_ctx *ctx, int32_t *count);
 
 /* ---- hot path ------------------------------------------------------------ */
-/* nSteps full steps, each stream + moments + collide fused in one kernel.
- * Replaces: the loop body of Radiation::RunSimulation (src/Radiation.cpp:795-819)
- * for StreamingType FlatFixed and CurvedFixed.  If want_moments is non-zero the
- * moments of the last step are kept for rt_get_moments. */
+/* nSteps full steps.  FlatFixed and CurvedFixed: stream + moments + collide
+ * fused in one kernel where rt_uses_fused_path says so, else the
+ * reference-order kernels; FlatAdaptive and CurvedAdaptive: rotation update,
+ * adaptive stream, rotated moments and collide (reference-order kernels).
+ * Replaces: the loop body of Radiation::RunSimulation (src/Radiation.cpp:784-832).
+ * If want_moments is non-zero the moments of the last step are kept for
+ * rt_get_moments (the reference-order kernels always keep them). */
 int rt_step(rt_ctx *ctx, int32_t nSteps, int32_t want_moments);
 /* Reference-order single methods (FP64 arithmetic, same operation order as
- * the reference), each followed by the reference's buffer swap.
- * Replace: Radiation::StreamFlatFixed / StreamCurvedFixed (:380-414, :454-512),
+ * the reference), each followed by the reference's buffer swap (for adaptive
+ * streaming rt_stream also swaps the two angle buffers).
+ * Replace: Radiation::StreamFlatFixed / StreamFlatAdaptive / StreamCurvedFixed /
+ * StreamCurvedAdaptive (:380-414, :415-453, :454-512, :513-575),
  * ComputeMomentsIF (:239-269) and Collide (:641-736). */
 int rt_stream(rt_ctx *ctx);
 int rt_compute_moments(rt_ctx *ctx);
 int rt_collide(rt_ctx *ctx);
+
+/* ---- adaptive streaming (FlatAdaptive, CurvedAdaptive) -------------------
+ * Every cell carries a rotation of the stencil, rotationAngle, and the angle
+ * of the next stream, rotationAngleNew (src/Radiation.h:42-43); both swap with
+ * every stream.  rt_step runs UpdateRotationMatrizes, the adaptive stream,
+ * ComputeMomentsIF and Collide in the rotated frames per step
+ * (src/Radiation.cpp:801-819).  Angles must lie in [-pi/2, 5pi/2], where the
+ * reference's MySin/MyCos are defined; a new context holds 0 in both. */
+/* Replaces: writes to Radiation::rotationAngle / rotationAngleNew
+ * (src/Radiation.cpp:55, 171, 177).  angleNew NULL: 0 everywhere. */
+int rt_set_rotation(rt_ctx *ctx, const double *angle, const double *angleNew);
+int rt_get_rotation(rt_ctx *ctx, double *angle, double *angleNew);
+/* Replaces: Radiation::UpdateRotationMatrizes (src/Radiation.cpp:361-378),
+ * from the moments the context holds (rt_set_moments, or the last step). */
+int rt_update_rotation(rt_ctx *ctx);
+/* Rotation angle LoadInitialData re-imposes on every source cell of
+ * rt_set_sources before each step (src/Radiation.cpp:171, 177): angle[N]
+ * read at the source cells, NaN for a cell whose angle is left alone (the
+ * reference skips cells with a vanishing initial flux).  Call after
+ * rt_set_sources; a new rt_set_sources clears it. */
+int rt_set_source_rotation(rt_ctx *ctx, const double *angle);
+/* Replaces: writes to Radiation::E, Fx, Fy, Pxx, Pxy, Pyy, which
+ * UpdateRotationMatrizes reads (AverageF, :345-359).  NULL entries are left. */
+int rt_set_moments(rt_ctx *ctx, const double *E, const double *Fx, const double *Fy,
+                   const double *Pxx, const double *Pxy, const double *Pyy);
 /* Which implementation rt_step uses: 1 fused kernel, 0 reference-order
  * kernels (for stencils or time steps the fused kernel does not cover). */
 int rt_uses_fused_path(rt_ctx *ctx, int32_t *fused);
@@ -252,6 +282,14 @@ int rt_build_flux_tables(int32_t nOrder, int32_t capacity, double *relativeFlux,
                          double *normalization, int32_t *count);
 int rt_build_stencil(int32_t nOrder, double *w, double *phi, double *cx, double *cy,
                      int32_t *Nc, double *Wc);
+/* Stencil(nOrder, nGhost, true) with nGhost weight-0 ghost directions near the
+ * stencil's forward direction (src/Stencil.cpp:177-297), sorted by angle
+ * (SortDirections, :299-332): w, phi, cx, cy get nOrder+nGhost entries and
+ * Nc/Wc the InterpolationGrid(10*nOrder) tables.  nGhost > 0 needs an even
+ * nOrder; layouts whose ghost counts do not add up (the reference then
+ * overruns its ghost array) are rejected. */
+int rt_build_stencil_ghost(int32_t nOrder, int32_t nGhost, double *w, double *phi, double *cx,
+                           double *cy, int32_t *Nc, double *Wc);
 /* Metric arrays of Minkowski (kind 0), SchwarzSchild (1) or KerrSchild (2)
  * on a reference Grid(nx_, ny_, start, end) (src/Metric.cpp:126-336):
  * fills the per-cell block [N*RT_NMETRIC] (metric may be NULL) and, if non-null,

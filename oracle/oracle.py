@@ -45,7 +45,8 @@ class Sim(C.Structure):
                 ("alpha", C.c_void_p), ("t11", C.c_void_p), ("t12", C.c_void_p),
                 ("t21", C.c_void_p), ("t22", C.c_void_p), ("bh_r2", C.c_double),
                 ("eta", C.c_void_p), ("kappa0", C.c_void_p), ("kappa1", C.c_void_p),
-                ("kappaA", C.c_void_p), ("ux", C.c_void_p), ("uy", C.c_void_p)]
+                ("kappaA", C.c_void_p), ("ux", C.c_void_p), ("uy", C.c_void_p),
+                ("rot", C.c_void_p), ("rotNew", C.c_void_p)]
 
 
 def build():
@@ -90,6 +91,16 @@ def lib(fast=False):
     L.or_moments.argtypes = [C.POINTER(Sim), dptr, dptr]
     L.or_moments_lf.argtypes = [C.POINTER(Grid), dptr, dptr, dptr, d, dptr]
     L.or_collide.argtypes = [C.POINTER(Sim), dptr, dptr, dptr, C.c_void_p]
+    L.or_myacos.restype = d
+    L.or_myacos.argtypes = [d]
+    L.or_stencil_ghost.restype = C.c_int
+    L.or_stencil_ghost.argtypes = [i32, i32, dptr, dptr, dptr, dptr]
+    L.or_update_rotation.argtypes = [C.POINTER(Sim), dptr, dptr, dptr]
+    for f in ("or_stream_flat_adaptive", "or_stream_curved_adaptive"):
+        getattr(L, f).argtypes = [C.POINTER(Sim), dptr, dptr]
+    L.or_simulate_adaptive.restype = C.c_int
+    L.or_simulate_adaptive.argtypes = [C.POINTER(Sim), i32, dptr, dptr, dptr, dptr, C.POINTER(i32),
+                                       C.POINTER(d), C.c_void_p]
     L.or_num_threads.restype = i32
     L.or_set_threads.argtypes = [i32]
     _bind_adjoint(L)
@@ -116,6 +127,16 @@ def make_grid(nx_, ny_, start, end, cfl):
 def stencil(n):
     w, phi, cx, cy = (np.zeros(n) for _ in range(4))
     lib().or_stencil(n, w, phi, cx, cy)
+    return w, phi, cx, cy
+
+
+def stencil_ghost(nOrder, nGhost):
+    """Stencil(nOrder, nGhost) sorted by angle: (w, phi, cx, cy), nOrder+nGhost each."""
+    n = nOrder + nGhost
+    w, phi, cx, cy = (np.zeros(n) for _ in range(4))
+    rc = lib().or_stencil_ghost(nOrder, nGhost, w, phi, cx, cy)
+    if rc != 0:
+        raise ValueError(f"Stencil({nOrder},{nGhost}) rejected ({rc})")
     return w, phi, cx, cy
 
 
@@ -154,12 +175,17 @@ def geodesic_tables(g, M, bh_r2, tol=1e-4, nF=5):
 class Problem:
     """Holds every array an oracle run needs (keeps them alive for ctypes)."""
 
-    def __init__(self, grid, nDir, streaming, M, bh_r2=-1.0, tables=None, fields=None, fast=False):
+    def __init__(self, grid, nDir, streaming, M, bh_r2=-1.0, tables=None, fields=None, fast=False, nGhost=0):
         self.L = lib(fast)
         self.grid = grid
         self.nDir = nDir
-        self.w, self.phi, self.cx, self.cy = stencil(nDir)
-        self.Nc, self.Wc = interp_grid(self.phi)
+        self.nOrder = nDir - nGhost
+        if nGhost:
+            self.w, self.phi, self.cx, self.cy = stencil_ghost(self.nOrder, nGhost)
+        else:
+            self.w, self.phi, self.cx, self.cy = stencil(nDir)
+        self.nGrid = 10 * self.nOrder
+        self.Nc, self.Wc = interp_grid(self.phi, self.nGrid)
         self.M = np.ascontiguousarray(M)
         N = grid.N
         self.alpha = np.ascontiguousarray(self.M[ALPHA])
@@ -172,7 +198,7 @@ class Problem:
         s.grid = grid
         s.nDir = nDir
         s.w, s.phi, s.cx, s.cy = (a.ctypes.data for a in (self.w, self.phi, self.cx, self.cy))
-        s.nGrid = 10 * nDir
+        s.nGrid = self.nGrid
         s.Nc, s.Wc = self.Nc.ctypes.data, self.Wc.ctypes.data
         s.streaming = streaming
         s.nCoeff = 5
@@ -219,6 +245,43 @@ class Problem:
         mom = np.zeros(6 * self.grid.N)
         self.L.or_moments(C.byref(self.sim), np.ascontiguousarray(I, dtype=np.float64), mom)
         return mom.reshape(6, self.grid.N)
+
+    def set_rotation(self, rot, rotNew=None):
+        """Rotation angles used by moments(), stream_adaptive() and collide."""
+        N = self.grid.N
+        self.rot = np.ascontiguousarray(rot, dtype=np.float64).copy()
+        self.rotNew = (np.zeros(N) if rotNew is None else np.ascontiguousarray(rotNew, dtype=np.float64).copy())
+        self.sim.rot = self.rot.ctypes.data
+        self.sim.rotNew = self.rotNew.ctypes.data
+
+    def update_rotation(self, mom, rot):
+        """UpdateRotationMatrizes: returns rotationAngleNew ([N]; halo = 0)."""
+        out = np.zeros(self.grid.N)
+        self.L.or_update_rotation(C.byref(self.sim), np.ascontiguousarray(mom, dtype=np.float64).reshape(-1),
+                                  np.ascontiguousarray(rot, dtype=np.float64), out)
+        return out
+
+    def stream_adaptive(self, I):
+        """One adaptive stream with the angles of set_rotation (no swap)."""
+        out = np.zeros(self.grid.N * self.nDir)
+        f = self.L.or_stream_curved_adaptive if self.sim.streaming == 3 else self.L.or_stream_flat_adaptive
+        f(C.byref(self.sim), np.ascontiguousarray(I, dtype=np.float64), out)
+        return out
+
+    def run_adaptive(self, I, nSteps, mom=None, rot=None, rotNew=None, want_counts=False):
+        """nSteps adaptive steps; returns (I, moments, rot, rotNew, max, sum, counts)."""
+        N = self.grid.N
+        I = np.ascontiguousarray(I, dtype=np.float64).copy()
+        mom = np.zeros(6 * N) if mom is None else np.ascontiguousarray(mom, dtype=np.float64).reshape(-1).copy()
+        rot = np.zeros(N) if rot is None else np.ascontiguousarray(rot, dtype=np.float64).copy()
+        rotNew = np.zeros(N) if rotNew is None else np.ascontiguousarray(rotNew, dtype=np.float64).copy()
+        mx = C.c_int32(0)
+        sm = C.c_double(0)
+        counts = np.zeros(nSteps * N, np.int32) if want_counts else None
+        rc = self.L.or_simulate_adaptive(C.byref(self.sim), nSteps, I, mom, rot, rotNew, C.byref(mx), C.byref(sm),
+                                         counts.ctypes.data if want_counts else None)
+        assert rc == 0
+        return I, mom.reshape(6, N), rot, rotNew, mx.value, sm.value, counts
 
     def run(self, I, nSteps, want_counts=False):
         """Advance I (AoS [N*nDir], float64) by nSteps; returns (I, moments, max, sum, counts)."""

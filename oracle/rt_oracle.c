@@ -663,9 +663,15 @@ void or_moments(const or_sim *s, const double *I, double *mom)
 #pragma omp parallel for schedule(dynamic, 50)
     for (long ij = 0; ij < N; ij++) {
         double e = 0, fx = 0, fy = 0, pxx = 0, pxy = 0, pyy = 0;
+        /* RotationMatrix(rotationAngle[ij]) * C(d) (:257) */
+        const double rs = s->rot ? or_mysin(s->rot[ij]) : 0.0, rc = s->rot ? or_mycos(s->rot[ij]) : 1.0;
         for (int d = 0; d < D; d++) {
             if (s->w[d] == 0.0) continue;
             double nx_ = s->cx[d], ny_ = s->cy[d];
+            if (s->rot) {
+                nx_ = s->cx[d] * rc - s->cy[d] * rs;
+                ny_ = s->cx[d] * rs + s->cy[d] * rc;
+            }
             double c = s->w[d] * I[ij * D + d];
             e += c;
             fx += c * nx_;
@@ -735,6 +741,7 @@ void or_collide(const or_sim *s, const double *I, double *Inew, double *mom, int
             double alpha = s->alpha[ij];
             double uu = u1 * u1 + u2 * u2;
             double lorentz = 1.0 / sqrt(1.0 - uu);
+            const double rs = s->rot ? or_mysin(s->rot[ij]) : 0.0, rc = s->rot ? or_mycos(s->rot[ij]) : 1.0;
             int cycles = 0;
             double diff = 1;
             while (fabs(diff) > 1e-4 && (cycles < 100)) {
@@ -751,6 +758,10 @@ void or_collide(const or_sim *s, const double *I, double *Inew, double *mom, int
                 for (int d = 0; d < D; d++) {
                     long idx = ij * D + d;
                     double n1 = s->cx[d], n2 = s->cy[d];
+                    if (s->rot) { /* nIF = RotationMatrix(rotationAngle[ij]) * C(d) (:691) */
+                        n1 = s->cx[d] * rc - s->cy[d] * rs;
+                        n2 = s->cx[d] * rs + s->cy[d] * rc;
+                    }
                     double un = u1 * n1 + u2 * n2;
                     double A = lorentz * (1.0 - un);
                     /* Tensor2 nFF = (nIF - (1 - (lorentz*un)/(lorentz+1)) * lorentz * uIF) / A */

@@ -76,13 +76,16 @@ typedef struct {
     int32_t nGrid;
     const int32_t *Nc;                 /* [nGrid*4] */
     const double *Wc;                  /* [nGrid*4] */
-    int32_t streaming;                 /* 0 FlatFixed, 2 CurvedFixed */
+    int32_t streaming;                 /* 0 FlatFixed, 1 FlatAdaptive, 2 CurvedFixed, 3 CurvedAdaptive */
     int32_t nCoeff;
     const double *cS, *cX, *cY, *cCx, *cCy; /* [N*nCoeff], curved only */
     const double *alpha;               /* [N] */
     const double *t11, *t12, *t21, *t22; /* [N] spatial tetrad (ul) */
     double bh_r2;                      /* excision threshold on r^2, <0: none */
     const double *eta, *kappa0, *kappa1, *kappaA, *ux, *uy; /* [N] */
+    /* rotationAngle / rotationAngleNew [N] (src/Radiation.h:42-43); rot NULL
+     * means 0 everywhere, as for fixed streaming */
+    const double *rot, *rotNew;
 } or_sim;
 
 /* Runs nSteps full steps (stream, moments, collide) in place on I (AoS).
@@ -100,6 +103,23 @@ void or_moments_lf(const or_grid *g, const double *mom /*[6][N]*/, const double 
                    const double *gamma /*[3][N]*/, double r2, double *out /*[7][N]*/);
 /* Collide in place on the moment arrays; writes Inew for interior active cells. */
 void or_collide(const or_sim *s, const double *I, double *Inew, double *mom, int32_t *count);
+
+/* ---- Adaptive streaming (rt_adaptive.c) ---- */
+double or_myacos(double x);                                /* src/Utility.hh:453-471 */
+void or_rotate(double angle, double x, double y, double *ox, double *oy); /* DataTypes.hh:630-635 */
+/* Stencil(nOrder, nGhost) (src/Stencil.cpp:177-332), sorted by angle; nOrder+nGhost
+ * entries each.  Returns 0, or -1 (bad sizes) / -2 (ghost counts do not add up). */
+int or_stencil_ghost(int32_t nOrder, int32_t nGhost, double *w, double *phi, double *cx, double *cy);
+/* UpdateRotationMatrizes (src/Radiation.cpp:345-378) */
+void or_update_rotation(const or_sim *s, const double *mom, const double *rot, double *rotNew);
+/* StreamFlatAdaptive / StreamCurvedAdaptive (:415-453, :513-575), using s->rot and s->rotNew */
+void or_stream_flat_adaptive(const or_sim *s, const double *I, double *Inew);
+void or_stream_curved_adaptive(const or_sim *s, const double *I, double *Inew);
+/* nSteps adaptive steps (s->streaming 1 or 3).  mom [6][N], rot, rotNew [N]
+ * are the reference's E..Pyy, rotationAngle, rotationAngleNew before the
+ * loop and are updated in place. */
+int or_simulate_adaptive(const or_sim *s, int32_t nSteps, double *I, double *mom, double *rot, double *rotNew,
+                         int32_t *maxIt, double *sumIt, int32_t *itcount);
 
 int32_t or_num_threads(void);
 void or_set_threads(int32_t n);

@@ -13,7 +13,14 @@
 //                                     frames and Log.txt under ../output/driver_frames/
 //   radiation_driver vonmises out.bin Minkowski 20x20 on [-1,1]^2, 16 directions, no steps:
 //                                     initial data given as lab-frame moments at three cells
-// out.bin: I (nxy*nDir doubles) then E, Fx, Fy (nxy doubles each).
+//   radiation_driver adaptive out.bin Minkowski 30x30 on [-1,1]^2, Stencil(16,4), FlatAdaptive,
+//                                     20 steps, kappa0 0.5, kappa1 0.2, three source cells
+//                                     (keepSourceNodesActive) with intensities 1 + 0.5 cx_d
+//                                     turned to the angles 0.3, 2.0, 4.5
+//   radiation_driver adaptive_curved out.bin  Schwarzschild 30x30 on [-4,4]^2, Stencil(16,4),
+//                                     CurvedAdaptive, 8 steps, eta = 1 on 3 < r < 3.5
+// out.bin: I (nxy*nDir doubles) then E, Fx, Fy (nxy doubles each); the
+// adaptive modes append rotationAngle (nxy doubles).
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -26,29 +33,34 @@ int main(int argc, char **argv)
         std::fprintf(stderr, "usage: %s flat|curved out.bin\n", argv[0]);
         return 2;
     }
-    const bool curved = std::strcmp(argv[1], "curved") == 0;
+    const bool adaptiveCurved = std::strcmp(argv[1], "adaptive_curved") == 0;
+    const bool adaptive = adaptiveCurved || std::strcmp(argv[1], "adaptive") == 0;
+    const bool curved = adaptiveCurved || std::strcmp(argv[1], "curved") == 0;
     const bool frames = std::strcmp(argv[1], "frames") == 0;
     const bool vonmises = std::strcmp(argv[1], "vonmises") == 0;
-    const size_t n = vonmises ? 20 : ((curved || frames) ? 40 : 100);
+    const size_t n = adaptive ? 30 : (vonmises ? 20 : ((curved || frames) ? 40 : 100));
     const double L = curved ? 4.0 : 1.0;
     Grid grid(n, n, Coord(-L, -L), Coord(L, L));
     grid.SetCFL(0.5);
     Minkowski flat(grid, 1.0, 0.0);
     SchwarzSchild bh(grid, 1.0, 0.0);
     Metric &metric = curved ? (Metric &)bh : (Metric &)flat;
-    Stencil stencil((curved || vonmises) ? 16 : 20);
+    Stencil stencil((curved || vonmises || adaptive) ? 16 : 20, adaptive ? 4 : 0);
     Stencil streamingStencil(5, 0, false);
 
+    const int steps = adaptiveCurved ? 8 : (adaptive ? 20 : (vonmises ? 0 : (curved ? 10 : (frames ? 12 : 100))));
     Config config = {
         .name = curved ? "driver_curved" : (frames ? "driver_frames" : "driver_flat"),
-        .simTime = (vonmises ? 0 : (curved ? 10 : (frames ? 12 : 100))) * grid.dt,
+        .simTime = steps * grid.dt,
         .writePeriod = frames ? 5 * grid.dt : 1e9,
+        .keepSourceNodesActive = adaptive && !curved,
         .writeData = frames,
         .printSetup = false,
         .printProgress = false,
         .printResults = true,
         .saveInitialData = frames,
-        .streamingType = curved ? StreamingType::CurvedFixed : StreamingType::FlatFixed,
+        .streamingType = adaptive ? (curved ? StreamingType::CurvedAdaptive : StreamingType::FlatAdaptive)
+                                  : (curved ? StreamingType::CurvedFixed : StreamingType::FlatFixed),
         .initialDataType = vonmises ? InitialDataType::Moments : InitialDataType::Intensities,
     };
     Radiation radiation(metric, stencil, streamingStencil, config);
@@ -64,10 +76,25 @@ int main(int argc, char **argv)
             radiation.initialFy_LF[cells[c]] = mom[c][2];
         }
     }
+    if (adaptive && !curved) {
+        const size_t cells[3] = {grid.Index(8, 8), grid.Index(20, 12), grid.Index(14, 22)};
+        const double angle[3] = {0.3, 2.0, 4.5};
+        for (int c = 0; c < 3; c++) {
+            radiation.isInitialGridPoint[cells[c]] = true;
+            radiation.initialFluxAngle_IF[cells[c]] = angle[c];
+            for (size_t d = 0; d < stencil.nDir; d++)
+                radiation.initialI[radiation.Index(cells[c], d)] = 1.0 + 0.5 * stencil.Cx(d);
+        }
+    }
     for (size_t j = 0; j < grid.ny; j++)
         for (size_t i = 0; i < grid.nx; i++) {
             const size_t ij = grid.Index(i, j);
             const double x = grid.x(i), y = grid.y(j), r = std::sqrt(x * x + y * y);
+            if (adaptive && !curved) {
+                radiation.kappa0[ij] = 0.5;
+                radiation.kappa1[ij] = 0.2;
+                continue;
+            }
             radiation.eta[ij] = curved ? ((r > 3 && r < 3.5) ? 1.0 : 0.0) : std::exp(-(x * x + y * y) / (2 * 0.1 * 0.1));
         }
     radiation.RunSimulation();
@@ -78,6 +105,7 @@ int main(int argc, char **argv)
     std::fwrite(radiation.E.data(), sizeof(double), grid.nxy, f);
     std::fwrite(radiation.Fx.data(), sizeof(double), grid.nxy, f);
     std::fwrite(radiation.Fy.data(), sizeof(double), grid.nxy, f);
+    if (adaptive) std::fwrite(radiation.rotationAngle.data(), sizeof(double), grid.nxy, f);
     std::fclose(f);
     std::printf("maxItterationCount %d\n", radiation.maxItterationCount);
     return 0;

@@ -46,6 +46,22 @@ def test_stencil_builder_matches_oracle(rt, O, n):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("n_order,n_ghost", [(4, 2), (16, 4), (32, 8), (40, 10), (80, 20), (160, 40)])
+def test_ghost_stencil_builder_matches_oracle(rt, O, n_order, n_ghost):
+    w, phi, cx, cy, Nc, Wc = rt.build_stencil_ghost(n_order, n_ghost)
+    ow, ophi, ocx, ocy = O.stencil_ghost(n_order, n_ghost)
+    oNc, oWc = O.interp_grid(ophi, 10 * n_order)
+    for a, b in ((w, ow), (phi, ophi), (cx, ocx), (cy, ocy), (Nc, oNc), (Wc, oWc)):
+        assert np.array_equal(a, b)
+
+
+def test_ghost_stencil_builder_rejections(rt):
+    with pytest.raises(rt.RtError):
+        rt.build_stencil_ghost(15, 3)   # odd nOrder
+    with pytest.raises(rt.RtError):
+        rt.build_stencil_ghost(16, -1)
+
+
 @pytest.mark.parametrize("kind", [0, 1, 2])
 def test_metric_builder_matches_oracle(rt, O, kind):
     gs = rt.GridSpec(37, 29, (-4, -3), (4, 3), 0.5)
@@ -78,7 +94,7 @@ def test_errors_are_reported_not_fatal(rt):
     bad = rt.Desc(2, 5, 0.1, 0.1, 0.05, 0.0, 0.0, 16, 5, rt.FP64, rt.FLAT_FIXED, -1.0, 0)
     assert L.rt_create(C.byref(bad), C.byref(h)) == rt.RT_ERR_ARG
     assert b"at least 2" in L.rt_last_error(None)
-    bad = rt.Desc(10, 10, 0.1, 0.1, 0.05, 0.0, 0.0, 16, 5, rt.FP64, rt.FLAT_ADAPTIVE, -1.0, 0)
+    bad = rt.Desc(10, 10, 0.1, 0.1, 0.05, 0.0, 0.0, 16, 5, rt.FP64, rt.GEODESIC_FIXED, -1.0, 0)
     assert L.rt_create(C.byref(bad), C.byref(h)) == rt.RT_ERR_UNSUPPORTED
     bad = rt.Desc(10, 10, -0.1, 0.1, 0.05, 0.0, 0.0, 16, 5, rt.FP64, rt.FLAT_FIXED, -1.0, 0)
     assert L.rt_create(C.byref(bad), C.byref(h)) == rt.RT_ERR_ARG

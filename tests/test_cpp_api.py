@@ -62,6 +62,56 @@ def test_radiation_curved_schwarzschild_matches_oracle(O, tmp_path):
     assert rel_l2(data[:N * D], Iref) <= 1e-12
 
 
+@pytest.mark.gpu
+def test_radiation_flat_adaptive_sources_matches_oracle(O, tmp_path):
+    """FlatAdaptive with Stencil(16,4) on Minkowski 30x30, 20 steps, three
+    source cells (keepSourceNodesActive, intensities 1 + 0.5 cx_d at the
+    angles 0.3, 2.0, 4.5), kappa0 0.5, kappa1 0.2: FP64 <= 1e-12, angles to 1e-12."""
+    data, out = run_driver("adaptive", tmp_path)
+    g = O.make_grid(30, 30, (-1, -1), (1, 1), 0.5)
+    N, D = g.N, 20
+    P = O.Problem(g, D, 1, O.metric(g, 0), fields={"kappa0": np.full(N, 0.5), "kappa1": np.full(N, 0.2)},
+                  nGhost=4)
+    cells = [8 + 8 * g.nx, 20 + 12 * g.nx, 14 + 22 * g.nx]
+    I0 = np.zeros((N, D))
+    rot = np.zeros(N)
+    for c, a in zip(cells, (0.3, 2.0, 4.5)):
+        I0[c] = 1.0 + 0.5 * P.cx
+        rot[c] = a
+    I, mom, rn = I0.reshape(-1).copy(), np.zeros((6, N)), np.zeros(N)
+    for _ in range(20):
+        I = I.reshape(N, D)
+        I[cells] = I0[cells]
+        rot[cells] = (0.3, 2.0, 4.5)
+        I, mom, rot, rn, _, _, _ = P.run_adaptive(I.reshape(-1), 1, mom=mom, rot=rot, rotNew=rn)
+    ND = N * D
+    assert rel_l2(data[:ND], I) <= 1e-12
+    for q in range(3):
+        assert rel_l2(data[ND + q * N:ND + (q + 1) * N], mom[q]) <= 1e-12, q
+    assert np.max(np.abs(data[ND + 3 * N:] - rot)) <= 1e-12
+    assert np.any(np.abs(rot) > 0.01)                        # the beams turned their neighbours' stencils
+
+
+@pytest.mark.gpu
+def test_radiation_curved_adaptive_matches_oracle(O, tmp_path):
+    """CurvedAdaptive with Stencil(16,4) on Schwarzschild 30x30 on [-4,4]^2,
+    8 steps, ring emission, GPU-built tables against the oracle's: <= 1e-12."""
+    data, _ = run_driver("adaptive_curved", tmp_path)
+    g = O.make_grid(30, 30, (-4, -4), (4, 4), 0.5)
+    N, D = g.N, 20
+    x = g.startx + (np.arange(N) % g.nx) * g.dx
+    y = g.starty + (np.arange(N) // g.nx) * g.dy
+    r = np.sqrt(x * x + y * y)
+    bh = 2.1 * 2.1
+    M = O.metric(g, 1)
+    P = O.Problem(g, D, 3, M, bh_r2=bh, tables=O.geodesic_tables(g, M, bh, 1e-4),
+                  fields={"eta": ((r > 3) & (r < 3.5)).astype(float)}, nGhost=4)
+    I, mom, rot = P.run_adaptive(np.zeros(N * D), 8)[:3]
+    assert rel_l2(data[:N * D], I) <= 1e-12
+    assert np.max(np.abs(data[N * D + 3 * N:] - rot)) <= 1e-12
+    assert np.any(rot != 0)
+
+
 def _format3(t):
     """The reference's Format(t, 3) (src/Utility.hh:511-540) of a float time."""
     return (" " if t >= 0 else "") + f"{float(np.float32(t)):.3f}"
