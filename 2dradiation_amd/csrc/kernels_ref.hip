@@ -166,8 +166,9 @@ __device__ inline double ref_intensity_at(const T *__restrict__ I, long N, long 
     double dd = nG * angle / (2.0 * rtm::kPi);
     if (!(dd >= 1e-8)) dd = 1e-8; /* NaN angle: keep the table reads in range */
     if (dd > nG - 1e-8) dd = nG - 1e-8;
-    long d0 = (long)floor(dd) % nGrid;
-    long d1 = (d0 + 1) % nGrid;
+    /* floor(dd) < nGrid, so the reference's "% nGrid" of d0 is the identity */
+    const int d0 = (int)floor(dd);
+    const int d1 = (d0 + 1 == nGrid) ? 0 : d0 + 1;
     double v0 = 0, v1 = 0;
     for (int q = 0; q < 4; q++) {
         v0 += Wc[4 * d0 + q] * (double)I[(long)Nc[4 * d0 + q] * N + k];

@@ -70,28 +70,34 @@ WORKLOADS = {
                                      spacetime="schwarzschild", excision=None, tol=1e-4, steps=2000),
     "schwarzschild_256sq_d32": dict(number=4, grid=(256, 256, (-4.0, -4.0), (4.0, 4.0)), cfl=0.5, nDir=32,
                                     spacetime="schwarzschild", excision=None, tol=1e-4, steps=500),
+    # the 1024^2 Schwarzschild problem streamed adaptively: Stencil(48,16),
+    # 64 directions (48 Fourier + 16 ghost), same fields as the fixed one
+    "schwarzschild_1024sq_adaptive_48_16": dict(number=6, fields_like=3, grid=(1024, 1024, (-8.0, -8.0), (8.0, 8.0)),
+                                                cfl=0.5, nDir=64, nOrder=48, nGhost=16, streaming="adaptive",
+                                                spacetime="schwarzschild", excision=None, tol=1e-4, steps=2000),
 }
 
 
 def fields(name, grid, seed=None):
     """Fluid fields of a workload as a dict of [N] float64 arrays."""
     spec = WORKLOADS[name]
-    seed = 1000 + spec["number"] if seed is None else seed
+    number = spec.get("fields_like", spec["number"])
+    seed = 1000 + number if seed is None else seed
     x, y = grid.coords()
     r = np.sqrt(x * x + y * y)
     N = grid.N
     f = {k: np.zeros(N) for k in ("eta", "kappa0", "kappa1", "kappaA", "ux", "uy")}
-    if spec["number"] == 1:
+    if number == 1:
         f["eta"] = np.exp(-(x * x + y * y) / (2 * 0.1 ** 2))
-    elif spec["number"] == 2:
+    elif number == 2:
         f["eta"] = ((r >= 3.0) & (r <= 3.5)).astype(np.float64)
         f["kappa0"] = np.exp(-1.0 + 0.5 * normal(seed, N, 0))
-    elif spec["number"] == 3:
+    elif number == 3:
         g = gaussian_random_field(seed, grid.nx, grid.ny, 0.5 / grid.dx, 0.5 / grid.dy, 0)
         f["eta"] = np.maximum(0.0, g)
         f["kappa0"] = 2.0 * uniform(seed, N, 10)
         f["kappa1"] = 0.3 * uniform(seed, N, 11)
-    elif spec["number"] == 4:
+    elif number == 4:
         # the point where gradients are taken: a strictly positive emissivity,
         # so no intensity sits on the IntensityAt clamp at 0 (a kink where
         # central differences and the adjoint legitimately disagree)
@@ -127,7 +133,8 @@ def excision_r2(spec, m=1.0):
 def make_context(name, precision, overrides=None, seed=None, grid_override=None):
     """Builds a ready-to-step Context for a workload through the library:
     stencil and metric from its C++ classes, geodesic tables on the GPU."""
-    from . import Context, build_stencil, build_metric, FLAT_FIXED, CURVED_FIXED
+    from . import (Context, build_stencil, build_stencil_ghost, build_metric, FLAT_FIXED, CURVED_FIXED,
+                   FLAT_ADAPTIVE, CURVED_ADAPTIVE)
     spec = dict(WORKLOADS[name])
     if overrides:
         spec.update(overrides)
@@ -135,8 +142,16 @@ def make_context(name, precision, overrides=None, seed=None, grid_override=None)
     kind = spacetime_kind(spec)
     curved = kind != 0
     r2 = excision_r2(spec)
-    ctx = Context(grid, spec["nDir"], CURVED_FIXED if curved else FLAT_FIXED, precision, r2)
-    w, phi, cx, cy, Nc, Wc = build_stencil(spec["nDir"])
+    adaptive = spec.get("streaming") == "adaptive"
+    if adaptive:
+        streaming = CURVED_ADAPTIVE if curved else FLAT_ADAPTIVE
+    else:
+        streaming = CURVED_FIXED if curved else FLAT_FIXED
+    ctx = Context(grid, spec["nDir"], streaming, precision, r2)
+    if adaptive:
+        w, phi, cx, cy, Nc, Wc = build_stencil_ghost(spec["nOrder"], spec["nGhost"])
+    else:
+        w, phi, cx, cy, Nc, Wc = build_stencil(spec["nDir"])
     ctx.set_stencil(w, phi, cx, cy, Nc, Wc)
     block, alpha, t11, t12, t21, t22 = build_metric(kind, 1.0, 0.0, grid)
     ctx.set_geometry(alpha, t11, t12, t21, t22)

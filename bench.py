@@ -217,6 +217,77 @@ def batch_leg(rt, steps=200, n=64):
     return out
 
 
+def adaptive_cpu_baseline(rt, name, budget_s=8.0, max_steps=10):
+    """The oracle's -O3 -ffast-math OpenMP build running the adaptive workload
+    from I = 0 on the host cores: one warm-up step, then as many steps as fit
+    in the budget."""
+    from oracle import oracle as O
+    spec = rt.workloads.WORKLOADS[name]
+    nx_, ny_, start, end = spec["grid"]
+    og = O.make_grid(nx_, ny_, start, end, spec["cfl"])
+    gs = rt.GridSpec(nx_, ny_, start, end, spec["cfl"])
+    kind = rt.workloads.spacetime_kind(spec)
+    r2 = rt.workloads.excision_r2(spec)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    L = O.lib(fast=True)
+    L.or_set_threads(threads)
+    M = O.metric(og, kind)
+    tabs = O.geodesic_tables(og, M, r2, spec.get("tol", 1e-4)) if kind else None
+    f = rt.workloads.fields(name, gs)
+    P = O.Problem(og, spec["nDir"], 3 if kind else 1, M, bh_r2=r2, tables=tabs, fields=f, fast=True,
+                  nGhost=spec["nGhost"])
+    t0 = time.perf_counter()
+    I, mom, rot, rn = P.run_adaptive(np.zeros(og.N * spec["nDir"]), 1)[:4]
+    per = time.perf_counter() - t0
+    n = int(max(1, min(max_steps, budget_s / max(per, 1e-6))))
+    t1 = time.perf_counter()
+    P.run_adaptive(I, n, mom=mom, rot=rot, rotNew=rn)
+    elapsed = time.perf_counter() - t1
+    act = int(np.count_nonzero(_active(gs, r2)))
+    return {"value": act * spec["nDir"] * n / elapsed, "unit": "cell*dir updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{name}, {n} timed steps after 1 warm-up step, oracle -O3 -ffast-math OpenMP "
+                      f"({threads} threads), {elapsed:.1f} s"}
+
+
+def adaptive_leg(rt, steps=40, warmup=3, cpu=True):
+    """CurvedAdaptive streaming of the 1024^2 Schwarzschild problem with
+    Stencil(48,16) (64 directions): rotation update, rotated stream, moments
+    and collide per step, FP32-mixed and FP64, both from I = 0 with all
+    angles 0; FP32-mixed is compared with FP64 after the timed steps."""
+    name = "schwarzschild_1024sq_adaptive_48_16"
+    spec = rt.workloads.WORKLOADS[name]
+    out = {"workload": name, "stencil": f"Stencil({spec['nOrder']},{spec['nGhost']})", "steps": steps}
+    states = {}
+    for prec, tag in ((rt.FP32_MIXED, "fp32_mixed"), (rt.FP64, "fp64")):
+        ctx, meta = rt.workloads.make_context(name, prec)
+        gs = meta["grid"]
+        act = int(np.count_nonzero(_active(gs, meta["excision_r2"])))
+        zero_i, zero_n = np.zeros(gs.N * spec["nDir"]), np.zeros(gs.N)
+        ctx.step(warmup)
+        ctx.upload_I(zero_i)
+        ctx.set_rotation(zero_n, zero_n)
+        ctx.set_moments([zero_n] * 6)
+        ctx.synchronize()
+        ms = ctx.time_steps(steps)
+        upd = act * spec["nDir"] * steps
+        b_i = 4 if prec == rt.FP32_MIXED else 8
+        # algorithmic bytes: I in and out, the fields, the tables and both angle buffers
+        B = algorithmic_bytes_per_update(spec["nDir"], b_i, b_i) + 2 * 8 / spec["nDir"]
+        rate = upd / (ms * 1e-3)
+        out[tag] = {"value": rate, "unit": "cell*dir updates/s", "ms_per_step": ms / steps,
+                    "hbm_frac": B * rate / HBM_PEAK, "bytes_per_update": B}
+        states[tag] = ctx.download_I()
+        rot = ctx.rotation()[0]
+        out[tag]["turned_cells"] = int(np.count_nonzero(rot))
+        ctx.close()
+    out["fp32_over_fp64"] = out["fp32_mixed"]["value"] / out["fp64"]["value"]
+    out["rel_l2_fp32_vs_fp64"] = rel_l2(states["fp32_mixed"], states["fp64"])
+    if cpu:
+        out["cpu_baseline"] = adaptive_cpu_baseline(rt, name)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +299,8 @@ def main():
                     help="only run a plain device copy of MIB mebibytes (HBM counter calibration)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-problems leg")
+    ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-streaming leg")
+    ap.add_argument("--only-adaptive", action="store_true", help="run only the adaptive-streaming leg")
     ap.add_argument("--ref-steps", type=int, default=200,
                     help="steps of the FP64 reference-order comparison leg (0 skips it)")
     args = ap.parse_args()
@@ -247,6 +320,9 @@ def main():
         return
     rt = importlib.import_module("2dradiation_amd")
     rt.lib()
+    if args.only_adaptive:
+        print(json.dumps({"adaptive": adaptive_leg(rt, cpu=not args.no_cpu)}), flush=True)
+        return
     spec = rt.workloads.WORKLOADS[WORKLOAD]
     K, W = args.steps, args.warmup
 
@@ -358,6 +434,9 @@ def main():
     if not args.no_batch:
         print("[bench] batch leg", file=sys.stderr, flush=True)
         result["batch"] = batch_leg(rt)
+    if not args.no_adaptive:
+        print("[bench] adaptive-streaming leg", file=sys.stderr, flush=True)
+        result["adaptive"] = adaptive_leg(rt, cpu=not args.no_cpu)
     if not args.no_cpu:
         print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(rt, WORKLOAD)
