@@ -7,26 +7,39 @@
 
 struct rank2Indices { int i, j; };
 
-struct Coord {
+// Two-component spatial vectors (Coord: positions, Tensor2: directions and
+// velocities) with the reference's 1-based components and componentwise
+// arithmetic (src/DataTypes.hh:37-213).
+template <class V>
+struct Vec2Base {
     double data[2];
-    Coord(double v = 0) { data[0] = data[1] = v; }
-    Coord(double x, double y) { data[0] = x; data[1] = y; }
     double &operator[](int k) { return data[k - 1]; }
     const double &operator[](int k) const { return data[k - 1]; }
     double EuklNormSquared() const { return data[0] * data[0] + data[1] * data[1]; }
     double EuklNorm() const { return std::sqrt(EuklNormSquared()); }
     double Phi() const { return rtm::my_phi(data[0], data[1]); }
+    V EuklNormalized() const
+    {
+        const double n = EuklNorm();
+        return V(data[0] / n, data[1] / n);
+    }
+    V operator+(const V &o) const { return V(data[0] + o.data[0], data[1] + o.data[1]); }
+    V operator-(const V &o) const { return V(data[0] - o.data[0], data[1] - o.data[1]); }
+    V operator-() const { return V(-data[0], -data[1]); }
+    V operator*(double s) const { return V(data[0] * s, data[1] * s); }
+    V operator/(double s) const { return V(data[0] / s, data[1] / s); }
+    friend V operator*(double s, const V &v) { return V(s * v.data[0], s * v.data[1]); }
+    static double Dot(const V &a, const V &b) { return a[1] * b[1] + a[2] * b[2]; }
+    static double Cross(const V &a, const V &b) { return a[1] * b[2] - b[1] * a[2]; }
+    friend std::ostream &operator<<(std::ostream &os, const V &v) { return os << "(" << v[1] << "," << v[2] << ")"; }
 };
-
-struct Tensor2 {
-    double data[2];
+struct Coord : Vec2Base<Coord> {
+    Coord(double v = 0) { data[0] = data[1] = v; }
+    Coord(double x, double y) { data[0] = x; data[1] = y; }
+};
+struct Tensor2 : Vec2Base<Tensor2> {
     Tensor2(double v = 0) { data[0] = data[1] = v; }
     Tensor2(double x, double y) { data[0] = x; data[1] = y; }
-    double &operator[](int k) { return data[k - 1]; }
-    const double &operator[](int k) const { return data[k - 1]; }
-    double EuklNorm() const { return std::sqrt(data[0] * data[0] + data[1] * data[1]); }
-    double Phi() const { return rtm::my_phi(data[0], data[1]); }
-    static double Dot(const Tensor2 &a, const Tensor2 &b) { return a[1] * b[1] + a[2] * b[2]; }
 };
 
 struct Tensor3 {
@@ -35,6 +48,15 @@ struct Tensor3 {
     Tensor3(double t, double x, double y) { data[0] = t; data[1] = x; data[2] = y; }
     double &operator[](int k) { return data[k]; }
     const double &operator[](int k) const { return data[k]; }
+    Tensor3 operator+(const Tensor3 &o) const { return Tensor3(data[0] + o[0], data[1] + o[1], data[2] + o[2]); }
+    Tensor3 operator-(const Tensor3 &o) const { return Tensor3(data[0] - o[0], data[1] - o[1], data[2] - o[2]); }
+    Tensor3 operator*(double s) const { return Tensor3(data[0] * s, data[1] * s, data[2] * s); }
+    Tensor3 operator/(double s) const { return Tensor3(data[0] / s, data[1] / s, data[2] / s); }
+    friend Tensor3 operator*(double s, const Tensor3 &v) { return Tensor3(s * v[0], s * v[1], s * v[2]); }
+    friend std::ostream &operator<<(std::ostream &os, const Tensor3 &v)
+    {
+        return os << "(" << v[0] << "," << v[1] << "," << v[2] << ")";
+    }
 };
 
 struct Tensor3x3 {
@@ -62,6 +84,18 @@ struct Tensor3x3 {
         return r;
     }
 };
+
+// Lorentz boost of velocity u (|u| < 1) as a 3x3 (t, x, y) matrix, the
+// identity for |u|^2 < 1e-10 (reference: BoostMatrix, src/SpecialMath.cpp:109-118).
+inline Tensor3x3 BoostMatrix(const Tensor2 &u)
+{
+    const double b2 = Tensor2::Dot(u, u);
+    const double g = 1.0 / std::sqrt(1.0 - b2);
+    if (b2 < 1e-10) return Tensor3x3(1, 0, 0, 0, 1, 0, 0, 0, 1);
+    const double k = (g - 1) / b2;
+    return Tensor3x3(g, -g * u[1], -g * u[2], -g * u[1], 1 + k * u[1] * u[1], k * u[1] * u[2], -g * u[2],
+                     k * u[1] * u[2], 1 + k * u[2] * u[2]);
+}
 
 // Sorted (input, output) table with clamped linear interpolation
 // (LookUpTable, src/DataTypes.hh:637-700): Add keeps the inputs increasing

@@ -183,3 +183,64 @@ void Metric::PackGeodesicBlock(double *out) const
         for (int q = 0; q < RT_NMETRIC; q++) out[ij * RT_NMETRIC + q] = (*src[q])[ij];
     });
 }
+
+// ---- values at a point -------------------------------------------------
+double Metric::InterpolateArrayTo_ij(const DoubleBuffer &array, const Coord &ij) const
+{
+    const size_t i0 = (size_t)std::floor(ij[1]), j0 = (size_t)std::floor(ij[2]);
+    const size_t i1 = i0 + 1, j1 = j0 + 1;
+    return rtm::bilinear<double>(ij[1] - i0, ij[2] - j0, array[grid.Index(i0, j0)], array[grid.Index(i0, j1)],
+                                 array[grid.Index(i1, j0)], array[grid.Index(i1, j1)]);
+}
+double Metric::GetAlpha(const Coord &xy) const
+{
+    return grid.OutsideDomain(xy) ? 1.0 : InterpolateArrayTo_ij(alpha, grid.ij(xy));
+}
+Tensor2 Metric::GetBeta_u(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor2(0.0);
+    const Coord ij = grid.ij(xy);
+    return Tensor2(InterpolateArrayTo_ij(beta1_u, ij), InterpolateArrayTo_ij(beta2_u, ij));
+}
+Tensor3x3 Metric::GetTetrad(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor3x3(1, 0, 0, 0, 1, 0, 0, 0, 1);
+    const Coord ij = grid.ij(xy);
+    const DoubleBuffer *e[9] = {&tetrad00_ul, &tetrad01_ul, &tetrad02_ul, &tetrad10_ul, &tetrad11_ul,
+                                &tetrad12_ul, &tetrad20_ul, &tetrad21_ul, &tetrad22_ul};
+    Tensor3x3 t;
+    for (int k = 0; k < 9; k++) t.data[k] = InterpolateArrayTo_ij(*e[k], ij);
+    return t;
+}
+Tensor3x3 Metric::GetTetradInverse(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor3x3(1, 0, 0, 0, 1, 0, 0, 0, 1);
+    const Coord ij = grid.ij(xy);
+    const DoubleBuffer *e[9] = {&tetrad00_lu, &tetrad01_lu, &tetrad02_lu, &tetrad10_lu, &tetrad11_lu,
+                                &tetrad12_lu, &tetrad20_lu, &tetrad21_lu, &tetrad22_lu};
+    Tensor3x3 t;
+    for (int k = 0; k < 9; k++) t.data[k] = InterpolateArrayTo_ij(*e[k], ij);
+    return t;
+}
+Tensor3x3 Metric::GetMetric_ll(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor3x3(-1, 0, 0, 0, 1, 0, 0, 0, 1);
+    const Coord ij = grid.ij(xy);
+    const double g00 = InterpolateArrayTo_ij(g00_ll, ij), g11 = InterpolateArrayTo_ij(g11_ll, ij),
+                 g22 = InterpolateArrayTo_ij(g22_ll, ij), g01 = InterpolateArrayTo_ij(g01_ll, ij),
+                 g02 = InterpolateArrayTo_ij(g02_ll, ij), g12 = InterpolateArrayTo_ij(g12_ll, ij);
+    return Tensor3x3(g00, g01, g02, g01, g11, g12, g02, g12, g22);
+}
+Tensor3 Metric::uEulObs(size_t ij) const
+{
+    const double a = GetAlpha(ij);
+    const Tensor2 b = GetBeta_u(ij);
+    return Tensor3(1.0 / a, -b[1] / a, -b[2] / a);
+}
+Tensor3 Metric::uEulObs(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor3(1, 0, 0);
+    const double a = GetAlpha(xy);
+    const Tensor2 b = GetBeta_u(xy);
+    return Tensor3(1.0 / a, -b[1] / a, -b[2] / a);
+}

@@ -43,6 +43,23 @@ class Metric {
     double GetAlpha(size_t ij) const { return alpha[ij]; }
     Tensor3x3 GetTetrad(size_t ij) const;
     Tensor3x3 GetTetradInverse(size_t ij) const;
+    Tensor2 GetBeta_u(size_t ij) const { return Tensor2(beta1_u[ij], beta2_u[ij]); }
+    Tensor3x3 GetMetric_ll(size_t ij) const
+    {
+        return Tensor3x3(g00_ll[ij], g01_ll[ij], g02_ll[ij], g01_ll[ij], g11_ll[ij], g12_ll[ij], g02_ll[ij], g12_ll[ij],
+                         g22_ll[ij]);
+    }
+    // Values at a point (reference src/Metric.cpp:338-359, 383-405, 541-608):
+    // bilinear in the cell arrays inside the domain, flat space outside.
+    double InterpolateArrayTo_ij(const DoubleBuffer &array, const Coord &ij) const;
+    double GetAlpha(const Coord &xy) const;
+    Tensor2 GetBeta_u(const Coord &xy) const;
+    Tensor3x3 GetTetrad(const Coord &xy) const;
+    Tensor3x3 GetTetradInverse(const Coord &xy) const;
+    Tensor3x3 GetMetric_ll(const Coord &xy) const;
+    // four-velocity of the Eulerian observer (src/Metric.cpp:368-382)
+    Tensor3 uEulObs(size_t ij) const;
+    Tensor3 uEulObs(const Coord &xy) const;
     // Per-cell block [nxy * RT_NMETRIC] consumed by rt_build_fourier_tables.
     void PackGeodesicBlock(double *out) const;
 

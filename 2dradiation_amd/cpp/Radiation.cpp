@@ -44,7 +44,8 @@ void Radiation::DownloadRotation()
 // reference) projected with Fourier::GetCoefficients
 // (src/FourierHarmonics.cpp:33-45).
 Radiation::Radiation(Metric &metric_, Stencil &stencil_, Stencil &streamingStencil_, Config config_)
-    : grid(metric_.grid), metric(metric_), stencil(stencil_), streamingStencil(streamingStencil_), config(config_)
+    : grid(metric_.grid), metric(metric_), stencil(stencil_), streamingStencil(streamingStencil_), config(config_),
+      logger(stencil_, streamingStencil_, metric_)
 {
     RequireSupportedStreaming();
     const size_t nxy = grid.nxy, D = stencil.nDir, nF = streamingStencil.nCoefficients;
@@ -337,66 +338,25 @@ void Radiation::Collide()
     averageItterationCount += sum;
 }
 
-// Log.txt of the reference's Logger::Write (src/Logger.hh:48-97), with the
-// quantities this library has: the stencil's interpolation-error and
-// sigma-table lines and the per-function profiler times are not produced.
-void Radiation::WriteLog(const std::string &directoryPath, const std::string &date, int timeSteps,
-                         double computeSeconds) const
-{
-    CreateDirectory(directoryPath);
-    std::ofstream file(directoryPath + "/Log.txt");
-    file << "Creation Date: " << date << std::endl << std::endl;
-    file << "Spacetime: " << metric.Name() << std::endl;
-    file << "Black Hole  Mass,   M = " << metric.m << std::endl;
-    file << "Black Hole  Spin,   a = " << metric.a << std::endl << std::endl;
-    file << "Grid Structure:" << std::endl;
-    file << "Simulation Time = " << config.simTime << std::endl;
-    file << "(startx,starty) = (" << grid.startx << "," << grid.starty << ")" << std::endl;
-    file << "(endx  ,  endy) = (" << grid.endx << "," << grid.endy << ")" << std::endl;
-    file << "(Nx    ,    Ny) = (" << grid.nx << "," << grid.ny << ")" << std::endl;
-    file << "Nt         = " << timeSteps << std::endl;
-    file << "dx         = " << grid.dx << std::endl;
-    file << "dy         = " << grid.dy << std::endl;
-    file << "dt         = " << grid.dt << std::endl;
-    file << "cfl        = " << grid.GetCFL() << std::endl << std::endl;
-    file << "Stencil Properties:" << std::endl;
-    file << "Intensity Stencil: nDir          = " << stencil.nDir << std::endl;
-    file << "Intensity Stencil: nOrder        = " << stencil.nOrder << std::endl;
-    file << "Intensity Stencil: nCoefficients = " << stencil.nCoefficients << std::endl;
-    file << "Streaming Stencil: nDir          = " << streamingStencil.nDir << std::endl;
-    file << "Streaming Stencil: nOrder        = " << streamingStencil.nOrder << std::endl;
-    file << "Streaming Stencil: nCoefficients = " << streamingStencil.nCoefficients << std::endl << std::endl;
-    file << "Lambda Itteration:" << std::endl;
-    file << "Max Lambda Itteration Count     = " << maxItterationCount << std::endl;
-    file << "Average Lambda Itteration Count = " << averageItterationCount << std::endl << std::endl;
-    file << "Time measurements:" << std::endl;
-    file << "Computation Time: " << computeSeconds << "s" << std::endl;
-    file << "Benchmark: " << grid.nxy * timeSteps / (1e6 * computeSeconds) << " MLUPS" << std::endl;
-}
-
 // Reference: src/Radiation.cpp:738-885.  The steps between two frames run on
 // the GPU as one rt_step call (stream, moments and collide fused per step).
 // A frame is written after the step at which the time since the last frame
 // reaches writePeriod - 1e-8, as in the reference, plus the initial frame
 // (saveInitialData) and a final one; each holds the lab-frame E, Fx, Fy and
 // |F| (ComputeMomentsIF, ComputeMomentsLF, Grid::WriteFrametoCsv) under
-// OUTPUTDIR/<name>/Moments/.  The reported computation time excludes frame
-// output, like the reference's.
+// OUTPUTDIR/<name>/Moments/.  `logger` gets the run's values and times: the
+// whole loop as "Total Time" and the CSV writes under the reference's frame
+// writer scope, so its computation time excludes frame output, like the
+// reference's; Log.txt is written with writeData.
 void Radiation::RunSimulation()
 {
     LoadInitialData();
     UpdateFourierCoefficients();
 
-    const int timeSteps = (int)std::ceil(config.simTime / grid.dt);
-    config.simTime = timeSteps * grid.dt;
-    const std::string directoryPath = OUTPUTDIR + config.name;
-    std::string date;
-    {
-        const std::time_t t = std::time(nullptr);
-        std::ostringstream oss;
-        oss << std::put_time(std::localtime(&t), "%H.%M.%S - %d.%m.%Y");
-        date = oss.str();
-    }
+    logger.SetValues(config.name, config.simTime);
+    config.simTime = logger.simTime;
+    const int timeSteps = logger.timeSteps;
+    const std::string directoryPath = logger.directoryPath;
     if (config.printSetup) {
         std::cout << " nx           = " << grid.nx << "\n";
         std::cout << " ny           = " << grid.ny << "\n";
@@ -431,12 +391,16 @@ void Radiation::RunSimulation()
         if (IsAdaptive() && any) Check(rt_set_source_rotation(ctx, sourceAngle.data()));
     }
     Check(rt_reset_iteration_stats(ctx));
-    double currentTime = config.t0, timeSinceLastFrame = 0, computeSeconds = 0;
+    using clock = std::chrono::steady_clock;
+    const auto loopStart = clock::now();
+    double currentTime = config.t0, timeSinceLastFrame = 0, computeSeconds = 0, writeSeconds = 0;
     auto frame = [&]() {
         Check(rt_compute_moments(ctx));
         DownloadMoments();
         DownloadMomentsLF();
+        const auto w0 = clock::now();
         grid.WriteFrametoCsv((float)currentTime, E_LF, Fx_LF, Fy_LF, F_LF, directoryPath + "/Moments/");
+        writeSeconds += std::chrono::duration<double>(clock::now() - w0).count();
         timeSinceLastFrame = 0;
     };
     if (config.writeData && config.saveInitialData) frame();
@@ -453,7 +417,7 @@ void Radiation::RunSimulation()
             writeNow = config.writeData && timeSinceLastFrame >= config.writePeriod - 1e-8;
         }
         const int last = (n + k == timeSteps) ? 1 : 0;
-        const auto t0 = std::chrono::steady_clock::now();
+        const auto t0 = clock::now();
         if (!perStep) {
             Check(rt_step(ctx, k, last));
         } else {
@@ -465,7 +429,7 @@ void Radiation::RunSimulation()
             }
         }
         Check(rt_synchronize(ctx));
-        computeSeconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        computeSeconds += std::chrono::duration<double>(clock::now() - t0).count();
         n += k;
         if (writeNow) frame();
     }
@@ -473,19 +437,25 @@ void Radiation::RunSimulation()
     if (config.writeData && timeSinceLastFrame != 0) frame();
     DownloadI();
     DownloadRotation();
+    const double totalSeconds = std::chrono::duration<double>(clock::now() - loopStart).count();
 
     int32_t mx = 0;
     double sum = 0;
     Check(rt_iteration_stats(ctx, &mx, &sum));
     maxItterationCount = mx;
     averageItterationCount = timeSteps > 0 ? sum / ((double)timeSteps * (grid.nx - 2) * (grid.ny - 2)) : 0.0;
-    std::cout << "total time: " << computeSeconds << "s" << std::endl;
+    logger.maxItterationCount = maxItterationCount;
+    logger.averageItterationCount = averageItterationCount;
+    logger.AddTimeMeasurement("Total Time", totalSeconds);
+    logger.AddTimeMeasurement(Logger::kFrameWriterScope, writeSeconds);
+    logger.AddTimeMeasurement("GPU steps (rt_step)", computeSeconds);
+    std::cout << "total time: " << totalSeconds << "s" << std::endl;
     if (config.printResults) {
         std::cout << "Max Lambda Itteration Count     = " << maxItterationCount << std::endl;
         std::cout << "Average Lambda Itteration Count = " << averageItterationCount << std::endl;
-        std::cout << "Computation Time: " << computeSeconds << "s" << std::endl;
-        std::cout << "Benchmark: " << grid.nxy * timeSteps / (1e6 * computeSeconds) << "MLUPS" << std::endl;
+        std::cout << "Computation Time: " << logger.ComputationTime() << "s" << std::endl;
+        std::cout << "Benchmark: " << grid.nxy * timeSteps / (1e6 * logger.ComputationTime()) << "MLUPS" << std::endl;
     }
-    if (config.writeData) WriteLog(directoryPath, date, timeSteps, computeSeconds);
+    if (config.writeData) logger.Write();
     if (config.keepSourceNodesActive) Check(rt_set_sources(ctx, nullptr, nullptr));
 }

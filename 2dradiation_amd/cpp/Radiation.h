@@ -21,7 +21,10 @@
 #ifndef RT2D_RADIATION_H
 #define RT2D_RADIATION_H
 #include "Config.hh"
+#include "Logger.hh"
 #include "Metric.h"
+#include "SpecialMath.h"
+#include "Spacetimes.h"
 #include "Stencil.h"
 
 struct rt_ctx;
@@ -42,8 +45,6 @@ class Radiation {
     void DownloadI();
     void DownloadMoments();
     void DownloadMomentsLF();
-    void WriteLog(const std::string &directoryPath, const std::string &date, int timeSteps,
-                  double computeSeconds) const;
     void RequireSupportedStreaming() const;
     bool IsAdaptive() const;
     bool IsCurved() const;
@@ -59,6 +60,7 @@ class Radiation {
     Stencil &stencil;
     Stencil &streamingStencil;
     Config config;
+    Logger logger;
     double sigmaOverwrite = -1;
 
     bool *isInitialGridPoint;

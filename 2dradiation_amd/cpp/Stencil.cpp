@@ -205,3 +205,20 @@ void Stencil::PopulateLookUpTable()
         }
     }
 }
+
+void Stencil::Print() const
+{
+    std::cout << name << ":\n";
+    std::cout << "           nDir: " << nDir << "\n";
+    std::cout << "         nGhost: " << nGhost << "\n";
+    std::cout << "       sigmaMax: " << sigmaMax << "\n";
+    std::cout << "relativeFluxMax: " << relativeFluxMax << "\n";
+}
+void Stencil::PrintAll() const
+{
+    std::cout << "        d,\t        w,\t      phi,\t       cx,\t       cy,\n";
+    for (size_t d = 0; d < nDir; d++)
+        std::cout << Format(d) << ",\t" << Format(W(d)) << ",\t" << Format(Phi(d)) << ",\t" << Format(Cx(d)) << ",\t"
+                  << Format(Cy(d)) << ",\n";
+    std::cout << std::endl;
+}

@@ -53,6 +53,9 @@ struct Stencil {
     const double *PhiData() const { return phi.data(); }
     const double *CxData() const { return cx.data(); }
     const double *CyData() const { return cy.data(); }
+    // Stencil::Print / PrintAll (reference src/Stencil.cpp:355-369)
+    void Print() const;
+    void PrintAll() const;
     void PopulateLookUpTable();
 };
 #endif

@@ -4,14 +4,22 @@
 #define RT2D_UTILITY_HH
 #include <cerrno>
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <filesystem>
+#include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <sstream>
 #include <string>
 #include <vector>
 #include "../csrc/rt_math.h"
+
+// PARALLEL_FOR(N) (src/ControlFlow.hh:23-24) for user loops: an OpenMP
+// collapse(N) loop when the user program is built with -fopenmp.
+#define RT2D_PRAGMA(x) _Pragma(#x)
+#define PARALLEL_FOR(N) RT2D_PRAGMA(omp parallel for collapse(N) schedule(dynamic, 50))
 
 // ExitOnError (src/Utility.hh:16-20): print and exit with errno.
 inline void ExitOnError(std::string msg = "")

@@ -17,6 +17,16 @@ def test_driver_is_built():
     assert os.path.exists(DRIVER), "tests/cpp/_build/radiation_driver missing: run __graft_entry__.build()"
 
 
+def test_host_api_helpers():
+    """Metric values at points, frame transforms, NullNormalize, the observed
+    velocity, boosts, vector arithmetic and Logger (tests/cpp/api_check.cpp);
+    host only."""
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "api_check")
+    assert os.path.exists(exe), "tests/cpp/_build/api_check missing: run __graft_entry__.build()"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "api_check OK" in r.stdout, r.stdout + r.stderr
+
+
 def run_driver(mode, tmp_path, cwd=None):
     out = str(tmp_path / f"{mode}.bin")
     print(f"[cpp-driver] {mode}", flush=True)
@@ -159,6 +169,9 @@ def test_radiation_frames_csv_and_log(O, tmp_path):
             assert np.allclose(vals[:, col], mom[q], rtol=2e-5, atol=1e-9 * max(1.0, np.abs(mom[q]).max())), (steps, q)
     log = (outdir / "Log.txt").read_text()
     assert "Spacetime: Minkowski" in log and f"Nt         = {total}" in log
+    # the reference Logger's stencil lines and time measurements
+    assert "Intensity Stencil: sigma max" in log and "Intensity Stencil: relative flux max" in log
+    assert "Total Time: " in log and "Computation Time: " in log and "MLUPS" in log
 
 
 @pytest.mark.gpu
