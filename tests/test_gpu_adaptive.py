@@ -95,6 +95,42 @@ def test_adaptive_steps_vs_oracle(rt, O, kind, n_order, n_ghost, n, steps):
     ctx.close()
 
 
+@pytest.mark.parametrize("kind,n_order,n_ghost,nx,ny", [(0, 8, 2, 37, 11), (2, 24, 6, 70, 5), (0, 120, 30, 9, 67)])
+def test_adaptive_ragged_shapes(rt, O, kind, n_order, n_ghost, nx, ny):
+    """Grids that are no multiple of the 64-cell workgroup strip and
+    direction counts that are no multiple of 4 waves (Stencil(8,2) = 10,
+    (24,6) = 30, (120,30) = 150 directions, the reference benchmark's
+    stencils)."""
+    print(f"[adaptive ragged] kind {kind}, Stencil({n_order},{n_ghost}), {nx}x{ny}", flush=True)
+    bh = 2.1 * 2.1 if kind else -1.0
+    span = 4.0 if kind else 1.0
+    gs, og = grids(rt, O, nx, ny, (-span, -span), (span, span))
+    D = n_order + n_ghost
+    fields = fields_for(gs.N, 17, u=(kind == 0))
+    M = O.metric(og, kind, 1.0, 0.0)
+    tables = O.geodesic_tables(og, M, bh, 1e-4) if kind else None
+    P = O.Problem(og, D, 3 if kind else 1, M, bh_r2=bh, tables=tables, fields=fields, nGhost=n_ghost)
+    ctx = rt.Context(gs, D, rt.CURVED_ADAPTIVE if kind else rt.FLAT_ADAPTIVE, rt.FP64, bh)
+    w, phi, cx, cy, Nc, Wc = rt.build_stencil_ghost(n_order, n_ghost)
+    ctx.set_stencil(w, phi, cx, cy, Nc, Wc)
+    block, alpha, t11, t12, t21, t22 = rt.build_metric(kind, 1.0, 0.0, gs)
+    ctx.set_geometry(alpha, t11, t12, t21, t22)
+    if kind:
+        ctx.set_streaming_stencil(*rt.build_stencil(5, tables=False))
+        ctx.set_fourier_tables(*tables)
+    ctx.set_fields(**fields)
+    I0, rot, rot_new, mom = initial(P, gs, 5 + D)
+    I, m, r, rn, mx, sm, _ = P.run_adaptive(I0, 5, mom=mom, rot=rot, rotNew=rot_new)
+    ctx.upload_I(I0)
+    ctx.set_rotation(rot, rot_new)
+    ctx.set_moments(mom)
+    ctx.step(5)
+    assert rel_l2(ctx.download_I(), I) <= 1e-12
+    assert rel_l2(ctx.moments(), m) <= 1e-12
+    assert np.max(np.abs(ctx.rotation()[0] - r)) <= 1e-12
+    ctx.close()
+
+
 def test_adaptive_single_methods_vs_oracle(rt, O):
     """UpdateRotationMatrizes, StreamCurvedAdaptive (with its swap of I and of
     the angle buffers), ComputeMomentsIF and Collide one by one."""
