@@ -179,6 +179,12 @@ int build_dir_tables(rt_ctx *c, const double *w, const double *phi, const double
 
 int refresh_departure_bound(rt_ctx *c)
 {
+    if (c->desc.streaming == RT_CURVED_ADAPTIVE && c->d_coefT && c->has_tables) {
+        /* relative-form series of the FP32 adaptive stream */
+        TRY(launch_coef_rel(c));
+        c->coefT_valid = true;
+        return RT_OK;
+    }
     if (c->desc.streaming != RT_CURVED_FIXED || !c->has_tables || !c->has_stencil) return RT_OK;
     double m = 0;
     TRY(launch_departure_bound(c, &m));
@@ -269,6 +275,10 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
         if ((r = dalloc(c, &c->d_rot[0], N * sizeof(double))) || (r = dalloc(c, &c->d_rot[1], N * sizeof(double))) ||
             (r = dalloc(c, &c->d_rtrig, 4 * N * sizeof(double))))
             return fail(r);
+        if (c->fp32 && (r = dalloc(c, &c->d_rtrigf, 4 * N * sizeof(float)))) return fail(r);
+        if (c->fp32 && d.streaming == RT_CURVED_ADAPTIVE && d.nCoeff == 5 &&
+            (r = dalloc(c, &c->d_coefT, 5 * (size_t)d.nCoeff * N * sizeof(float))))
+            return fail(r);
         /* rotationAngle = rotationAngleNew = 0 (src/Radiation.cpp:55) */
         if (hipMemsetAsync(c->d_rot[0], 0, N * sizeof(double), c->stream) != hipSuccess ||
             hipMemsetAsync(c->d_rot[1], 0, N * sizeof(double), c->stream) != hipSuccess)
@@ -302,6 +312,7 @@ void rt_destroy(rt_ctx *c)
     dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_coefT); dfree(c->d_excl); dfree(c->d_red);
     dfree(c->d_frame); dfree(c->d_lf); dfree(c->d_src_idx); dfree(c->d_src_I); dfree(c->d_metric);
     dfree(c->d_rot[0]); dfree(c->d_rot[1]); dfree(c->d_rtrig); dfree(c->d_src_rot);
+    dfree(c->d_rtrigf); dfree(c->d_Wcf);
     dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
     dfree(c->d_lam[0]); dfree(c->d_lam[1]); dfree(c->d_grad[0]); dfree(c->d_grad[1]); dfree(c->d_pts); dfree(c->d_mask); dfree(c->d_f64);
     dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_mom); dfree(c->d_count);
@@ -353,6 +364,15 @@ int rt_set_stencil(rt_ctx *c, const double *w, const double *phi, const double *
         if (!c->d_Wc) TRY(dalloc(c, &c->d_Wc, 4 * (size_t)nGrid * sizeof(double)));
         RT_HIP(c, hipMemcpyAsync(c->d_Nc, Nc, 4 * (size_t)nGrid * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
         RT_HIP(c, hipMemcpyAsync(c->d_Wc, Wc, 4 * (size_t)nGrid * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        if (c->fp32 && rt_is_adaptive(c->desc)) {
+            std::vector<float> wf(4 * (size_t)nGrid);
+            for (size_t k = 0; k < wf.size(); k++) wf[k] = (float)Wc[k];
+            dfree(c->d_Wcf);
+            c->d_Wcf = nullptr;
+            TRY(dalloc(c, &c->d_Wcf, wf.size() * sizeof(float)));
+            RT_HIP(c, hipMemcpyAsync(c->d_Wcf, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+            RT_HIP(c, hipStreamSynchronize(c->stream));
+        }
     }
     TRY(build_dir_tables(c, w, phi, cx, cy, Nc, Wc));
     c->has_stencil = true;

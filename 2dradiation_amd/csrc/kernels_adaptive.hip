@@ -21,7 +21,7 @@
  * coefficients in registers for all of its directions; stores of one
  * direction are coalesced over the 64 cells.
  */
-#include "rt_internal.h"
+#include "fused_common.h"
 
 namespace {
 
@@ -32,8 +32,8 @@ constexpr double kTwoPi = 2.0 * rtm::kPi;
  * in the reference. */
 template <bool UPDATE>
 __global__ __launch_bounds__(256) void k_rotation(const double *__restrict__ mom, const double *__restrict__ rot,
-                                                  double *__restrict__ rotNew, double *__restrict__ trig, int nx,
-                                                  int ny)
+                                                  double *__restrict__ rotNew, double *__restrict__ trig,
+                                                  float *__restrict__ trigf, int nx, int ny)
 {
     const long N = (long)nx * ny;
     const long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -65,6 +65,8 @@ __global__ __launch_bounds__(256) void k_rotation(const double *__restrict__ mom
     const double inv = fmod(-rot[ij] + kTwoPi, kTwoPi);
     trig[2 * N + ij] = rtm::my_sin(inv);
     trig[3 * N + ij] = rtm::my_cos(inv);
+    if (trigf)
+        for (int q = 0; q < 4; q++) trigf[q * N + ij] = (float)trig[q * N + ij];
 }
 
 /* IntensityAt (src/Radiation.cpp:322-343) of cell k: the inertial-frame
@@ -235,6 +237,157 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT_ADAPTIVE
     }
 }
 
+/* ---- FP32 arithmetic (FP32-mixed contexts) ----------------------------
+ * The same stream with the per-(cell, direction) work in FP32: rotations
+ * from FP32 copies of the sines and cosines, the Hastings angle as one
+ * branch-free index (octant offsets exact), one sincosf per direction, the
+ * Fourier series in the relative form (S - 1, departure offsets in cells,
+ * velocity), the lapse ratio as alpha_c^3 * alpha_k^-3, bilinear and angular
+ * interpolation as lerps.  Departure excision is tested in FP64 on absolute
+ * coordinates.  Offsets are 32-bit (N * D < 2^32). */
+struct AdaptiveArgsF {
+    const float *trigf;   /* [4][N] */
+    const double *cx, *cy;
+    const int32_t *Nc;
+    const float *Wcf;     /* [nGrid][4] */
+    int nGrid;
+    const float *coefT;   /* [25][N]: S - 1, X, Y offsets in cells, vx, vy (5 each) */
+    const float *ia3;     /* [N] alpha^-3 */
+    int nx, ny, D;
+    float dtdx, dtdy;     /* flat departure offset per unit velocity, in cells */
+    float fine_u;         /* nGrid / (2 pi) */
+    double startx, starty, dx, dy, r2;
+};
+
+/* IntensityAt (src/Radiation.cpp:322-343) in FP32: inverse rotation, fine
+ * angular index, two 4-tap cubic sums, lerp, clamp at 0. */
+__device__ inline float intensity_at_rot_f(const float *__restrict__ I, uint32_t N, uint32_t k, float vx, float vy,
+                                           float si, float ci, const int32_t *__restrict__ Nc,
+                                           const float *__restrict__ Wcf, int nGrid, float u, float q1, float q2,
+                                           float q3, float q4)
+{
+#pragma clang fp contract(fast)
+    const float x = vx * ci - vy * si, y = vx * si + vy * ci;
+    float dd = rtf::angle_index_bf<float>(x, y, u, q1, q2, q3, q4);
+    if (!(dd >= 0.0f)) dd = 0.0f; /* zero vector (angle 0) and NaN */
+    int d0 = (int)dd;
+    if (d0 > nGrid - 1) d0 = nGrid - 1;
+    const float t = fminf(dd - (float)d0, 1.0f);
+    const int d1 = (d0 + 1 == nGrid) ? 0 : d0 + 1;
+    const int4 n0 = *(const int4 *)(Nc + 4 * d0), n1 = *(const int4 *)(Nc + 4 * d1);
+    const float4 w0 = *(const float4 *)(Wcf + 4 * d0), w1 = *(const float4 *)(Wcf + 4 * d1);
+    float v0 = w0.x * I[(uint32_t)n0.x * N + k];
+    float v1 = w1.x * I[(uint32_t)n1.x * N + k];
+    v0 = fmaf(w0.y, I[(uint32_t)n0.y * N + k], v0);
+    v1 = fmaf(w1.y, I[(uint32_t)n1.y * N + k], v1);
+    v0 = fmaf(w0.z, I[(uint32_t)n0.z * N + k], v0);
+    v1 = fmaf(w1.z, I[(uint32_t)n1.z * N + k], v1);
+    v0 = fmaf(w0.w, I[(uint32_t)n0.w * N + k], v0);
+    v1 = fmaf(w1.w, I[(uint32_t)n1.w * N + k], v1);
+    return rtf::tmax0(fmaf(t, v1 - v0, v0));
+}
+
+/* departure offset d (cells) -> base cell and weight, clamped into the grid
+ * as the FP64 kernels clamp the departure coordinate */
+__device__ inline void split_offset(int i, float d, int n, int &i0, int &i1, float &w)
+{
+    const float f = floorf(d);
+    i0 = i + (int)f;
+    w = d - f;
+    if (!(d == d) || i0 < 0) {
+        i0 = 0;
+        w = 0.0f;
+    }
+    if (i0 >= n - 1) {
+        i0 = n - 1;
+        w = 0.0f;
+    }
+    i1 = i0 + 1 < n ? i0 + 1 : n - 1;
+}
+
+template <bool CURVED>
+__global__ __launch_bounds__(256) void k_stream_adaptive_f32(const float *__restrict__ I, float *__restrict__ Inew,
+                                                             AdaptiveArgsF a)
+{
+#pragma clang fp contract(fast)
+    const int nix = a.nx - 2;
+    const uint32_t Ni = (uint32_t)nix * (uint32_t)(a.ny - 2);
+    const uint32_t c = blockIdx.x * 64u + threadIdx.x;
+    if (c >= Ni) return;
+    const int i = 1 + (int)(c % nix), j = 1 + (int)(c / nix);
+    const uint32_t N = (uint32_t)a.nx * (uint32_t)a.ny, ij = (uint32_t)i + (uint32_t)j * (uint32_t)a.nx;
+    const float rs = a.trigf[ij], rc = a.trigf[N + ij];
+    const float *__restrict__ si = a.trigf + 2 * N;
+    const float *__restrict__ ci = a.trigf + 3 * N;
+    const float u = a.fine_u, q1 = 0.25f * a.nGrid, q2 = 0.5f * a.nGrid, q3 = 0.75f * a.nGrid, q4 = (float)a.nGrid;
+    const float hp = 1.5707963267948966f, pi = 3.14159265358979323846f, tp = 4.71238898038469f,
+                twopi = 6.283185307179586f;
+    const double xc = a.startx + i * a.dx, yc = a.starty + j * a.dy;
+    float cf[25];
+    float a3c = 1.0f;
+    if (CURVED) {
+        if (a.r2 >= 0.0 && xc * xc + yc * yc <= a.r2) {
+            for (int d = threadIdx.y; d < a.D; d += 4) Inew[(uint32_t)d * N + ij] = 0.0f;
+            return;
+        }
+#pragma unroll
+        for (int q = 0; q < 25; q++) cf[q] = a.coefT[(size_t)q * N + ij];
+        a3c = 1.0f / a.ia3[ij];
+    }
+    for (int d = threadIdx.y; d < a.D; d += 4) {
+        const float cxd = (float)a.cx[d], cyd = (float)a.cy[d];
+        const float ux = cxd * rc - cyd * rs, uy = cxd * rs + cyd * rc;
+        float di, dj, vx = ux, vy = uy, sh = 1.0f;
+        if (!CURVED) {
+            di = -ux * a.dtdx;
+            dj = -uy * a.dtdy;
+        } else {
+            const float ang = rtf::angle_index_bf<float>(ux, uy, 1.0f, hp, pi, tp, twopi);
+            float s1, c1;
+            sincosf(ang, &s1, &c1);
+            const float r2 = 1.4142135623730951f;
+            const float B[5] = {1.0f, r2 * c1, r2 * s1, r2 * (c1 * c1 - s1 * s1), r2 * (2.0f * s1 * c1)};
+            float q[5];
+#pragma unroll
+            for (int s = 0; s < 5; s++) {
+                float acc = cf[5 * s];
+#pragma unroll
+                for (int k = 1; k < 5; k++) acc = fmaf(cf[5 * s + k], B[k], acc);
+                q[s] = acc;
+            }
+            sh = 1.0f + q[0];
+            di = q[1];
+            dj = q[2];
+            vx = q[3];
+            vy = q[4];
+            const double xT = xc + a.dx * (double)di, yT = yc + a.dy * (double)dj;
+            if (a.r2 >= 0.0 && xT * xT + yT * yT <= a.r2) {
+                Inew[(uint32_t)d * N + ij] = 0.0f;
+                continue;
+            }
+        }
+        int i0, i1, j0, j1;
+        float tx, ty;
+        split_offset(i, di, a.nx, i0, i1, tx);
+        split_offset(j, dj, a.ny, j0, j1, ty);
+        const uint32_t k00 = (uint32_t)(i0 + j0 * a.nx), k01 = (uint32_t)(i0 + j1 * a.nx),
+                       k10 = (uint32_t)(i1 + j0 * a.nx), k11 = (uint32_t)(i1 + j1 * a.nx);
+        float f00 = intensity_at_rot_f(I, N, k00, vx, vy, si[k00], ci[k00], a.Nc, a.Wcf, a.nGrid, u, q1, q2, q3, q4);
+        float f01 = intensity_at_rot_f(I, N, k01, vx, vy, si[k01], ci[k01], a.Nc, a.Wcf, a.nGrid, u, q1, q2, q3, q4);
+        float f10 = intensity_at_rot_f(I, N, k10, vx, vy, si[k10], ci[k10], a.Nc, a.Wcf, a.nGrid, u, q1, q2, q3, q4);
+        float f11 = intensity_at_rot_f(I, N, k11, vx, vy, si[k11], ci[k11], a.Nc, a.Wcf, a.nGrid, u, q1, q2, q3, q4);
+        if (CURVED) {
+            /* (alpha_c / alpha_k)^3 = alpha_c^3 * alpha_k^-3 */
+            f00 *= a3c * a.ia3[k00];
+            f01 *= a3c * a.ia3[k01];
+            f10 *= a3c * a.ia3[k10];
+            f11 *= a3c * a.ia3[k11];
+        }
+        const float b = rtf::bilinear_lerp(tx, ty, f00, f01, f10, f11);
+        Inew[(uint32_t)d * N + ij] = CURVED ? (sh * (sh * sh)) * b : b;
+    }
+}
+
 /* Rotation angles re-imposed on source cells (LoadInitialData,
  * src/Radiation.cpp:171, 177); NaN marks a cell whose angle is left alone. */
 __global__ void k_restore_source_rot(double *__restrict__ rot, const int32_t *__restrict__ idx,
@@ -274,9 +427,11 @@ int launch_rotation(rt_ctx *c, bool update)
     const double *rot = c->d_rot[c->rcur];
     double *rotNew = c->d_rot[1 - c->rcur];
     if (update)
-        k_rotation<true><<<nblk(c->N, 256), 256, 0, c->stream>>>(c->d_mom, rot, rotNew, c->d_rtrig, d.nx, d.ny);
+        k_rotation<true><<<nblk(c->N, 256), 256, 0, c->stream>>>(c->d_mom, rot, rotNew, c->d_rtrig, c->d_rtrigf, d.nx,
+                                                                 d.ny);
     else
-        k_rotation<false><<<nblk(c->N, 256), 256, 0, c->stream>>>(c->d_mom, rot, rotNew, c->d_rtrig, d.nx, d.ny);
+        k_rotation<false><<<nblk(c->N, 256), 256, 0, c->stream>>>(c->d_mom, rot, rotNew, c->d_rtrig, c->d_rtrigf, d.nx,
+                                                                  d.ny);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }
@@ -306,6 +461,33 @@ int launch_stream_adaptive(rt_ctx *c, const void *in, void *out)
     a.r2 = d.excision_r2;
     const size_t Ni = (size_t)(d.nx - 2) * (d.ny - 2);
     const unsigned blocks = nblk(Ni, 64);
+    if (adaptive_f32_ready(c)) {
+        AdaptiveArgsF f;
+        f.trigf = c->d_rtrigf;
+        f.cx = c->d_cx;
+        f.cy = c->d_cy;
+        f.Nc = c->d_Nc;
+        f.Wcf = c->d_Wcf;
+        f.nGrid = c->nGrid;
+        f.coefT = (const float *)c->d_coefT;
+        f.ia3 = (const float *)c->d_cell + (size_t)CF_IA3 * c->N;
+        f.nx = d.nx;
+        f.ny = d.ny;
+        f.D = c->D;
+        f.dtdx = (float)(d.dt / d.dx);
+        f.dtdy = (float)(d.dt / d.dy);
+        f.fine_u = (float)(c->nGrid / (2.0 * rtm::kPi));
+        f.startx = d.startx;
+        f.starty = d.starty;
+        f.dx = d.dx;
+        f.dy = d.dy;
+        f.r2 = d.excision_r2;
+        const dim3 grid(blocks), block(64, 4);
+        if (curved) k_stream_adaptive_f32<true><<<grid, block, 0, c->stream>>>((const float *)in, (float *)out, f);
+        else k_stream_adaptive_f32<false><<<grid, block, 0, c->stream>>>((const float *)in, (float *)out, f);
+        RT_HIP(c, hipGetLastError());
+        return RT_OK;
+    }
     if (c->fp32) {
         if (curved) launch_stream_t<float, true>(c, in, out, a, blocks);
         else launch_stream_t<float, false>(c, in, out, a, blocks);
@@ -315,6 +497,15 @@ int launch_stream_adaptive(rt_ctx *c, const void *in, void *out)
     }
     RT_HIP(c, hipGetLastError());
     return RT_OK;
+}
+
+/* FP32-mixed contexts stream in FP32 when the FP32 tables exist, the
+ * offsets fit 32 bits and (curved) the series have 5 coefficients. */
+bool adaptive_f32_ready(const rt_ctx *c)
+{
+    if (!c->fp32 || !c->d_rtrigf || !c->d_Wcf || (size_t)c->N * c->D >= ((size_t)1 << 32)) return false;
+    if (c->desc.streaming == RT_CURVED_ADAPTIVE) return c->nCoeff == 5 && c->d_coefT && c->coefT_valid;
+    return true;
 }
 
 int launch_restore_source_rot(rt_ctx *c)

@@ -337,7 +337,7 @@ __global__ void k_coef_rel(const double *__restrict__ coef, T *__restrict__ out,
 
 int launch_coef_rel(rt_ctx *c)
 {
-    if (c->desc.streaming != RT_CURVED_FIXED || c->nCoeff != 5) return RT_OK;
+    if (!rt_is_curved(c->desc) || c->nCoeff != 5 || !c->d_coefT) return RT_OK;
     const rt_desc &d = c->desc;
     const unsigned nb = (unsigned)((c->N + 255) / 256);
     if (c->fp32)

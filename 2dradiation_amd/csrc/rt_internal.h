@@ -111,6 +111,11 @@ struct rt_ctx {
     double *d_rot[2] = {nullptr, nullptr};
     int rcur = 0;
     double *d_rtrig = nullptr;
+    /* FP32-mixed adaptive contexts: FP32 copies of d_rtrig and of the cubic
+     * weights, and (curved) d_coefT holding the relative-form series */
+    float *d_rtrigf = nullptr;
+    float *d_Wcf = nullptr;
+    bool coefT_valid = false;
     double *d_src_rot = nullptr;
     bool src_rot_set = false;
     int32_t *d_count = nullptr;   /* [N] per-method collide cycle counts */
@@ -162,6 +167,7 @@ int launch_ref_stream(rt_ctx *c, const void *in, void *out);
 int launch_rotation(rt_ctx *c, bool update);
 int launch_stream_adaptive(rt_ctx *c, const void *in, void *out);
 int launch_restore_source_rot(rt_ctx *c);
+bool adaptive_f32_ready(const rt_ctx *c);
 inline bool rt_is_adaptive(const rt_desc &d) { return d.streaming == RT_FLAT_ADAPTIVE || d.streaming == RT_CURVED_ADAPTIVE; }
 inline bool rt_is_curved(const rt_desc &d) { return d.streaming == RT_CURVED_FIXED || d.streaming == RT_CURVED_ADAPTIVE; }
 int launch_ref_moments(rt_ctx *c, const void *in);
