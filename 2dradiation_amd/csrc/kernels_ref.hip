@@ -311,6 +311,32 @@ __global__ void k_ref_collide(const T *__restrict__ I, T *__restrict__ Inew, dou
                      (pFy - lorentz * pE * u2 - uyP +
                       u2 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
         double e = 0, fx = 0, fy = 0, pxx = 0, pxy = 0, pyy = 0;
+        if (u1 == 0.0 && u2 == 0.0) {
+            /* Fluid at rest: lorentz = 1, u.n = 0, A = 1, fac = 1 and
+             * nFF = n exactly, so dropping the factors and divisions by 1
+             * below leaves every rounded value of the general form unchanged
+             * (up to the sign of zeros). */
+            const double adt = al * dt, den = 1.0 + al * dt * (kA + k0);
+            for (int d = 0; d < D; d++) {
+                const long idx = (long)d * N + ij;
+                double n1 = cx[d], n2 = cy[d];
+                if (rot) {
+                    n1 = cx[d] * rc - cy[d] * rs;
+                    n2 = cx[d] * rs + cy[d] * rc;
+                }
+                const double nF = n1 * fFx + n2 * fFy;
+                const double M = k0 * fE + 3.0 * k1 * nF;
+                const double v = ((double)I[idx] + adt * (eta + M)) / den;
+                Inew[idx] = (T)v;
+                const double c = w[d] * v;
+                e += c;
+                fx += c * n1;
+                fy += c * n2;
+                pxx += c * n1 * n1;
+                pxy += c * n1 * n2;
+                pyy += c * n2 * n2;
+            }
+        } else
         for (int d = 0; d < D; d++) {
             long idx = (long)d * N + ij;
             double n1 = cx[d], n2 = cy[d];
