@@ -350,7 +350,8 @@ def main():
         "metric": "cell*dir updates/s FP32-mixed vs FP64 at rel-L2<=1e-5; adjoint/forward ratio",
         "value": rate32, "unit": "cell*dir updates/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": ms32 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 intensities, fp64 tables and moment sums", "data": "synthetic (seeded fields)",
+        "dtype": "f32", "dtype_detail": "f32 intensities and gather arithmetic; f64 tables, moment sums and lambda iteration",
+        "data": "synthetic (seeded fields)",
         "config": {"workload": WORKLOAD, "grid": "1026x1026 stored (1024x1024 interior) on [-8,8]^2",
                    "nDir": spec["nDir"], "spacetime": "Schwarzschild M=1, excision r<=2.1",
                    "cfl": spec["cfl"], "steps_per_run": K, "active_cells": act, "parallelism": "single GPU"},
