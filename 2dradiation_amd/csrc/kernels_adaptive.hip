@@ -4,9 +4,12 @@
  *
  * Reference: UpdateRotationMatrizes / AverageF (src/Radiation.cpp:345-378),
  * StreamFlatAdaptive (:415-453), StreamCurvedAdaptive (:513-575), IntensityAt
- * (:322-343), RotationMatrix (src/DataTypes.hh:620-635).  FP64 arithmetic in
- * the reference's operation order; this file is built without FP
- * contraction.  The state may be stored in FP32 (FP32-mixed contexts).
+ * (:322-343), RotationMatrix (src/DataTypes.hh:620-635).  The rotation
+ * update keeps the reference's FP64 operation order (this file is built
+ * without FP contraction; its threshold is discontinuous).  The FP64 stream
+ * follows the reference's order with a*b+c contracted into FMAs (every value
+ * it rounds feeds a continuous function).  FP32-mixed contexts (FP32 state)
+ * stream with FP32 arithmetic in k_stream_adaptive_f32.
  *
  * Layout: intensities SoA [D][N] as everywhere in the library; rotation
  * angles [N] in two buffers that swap like the reference's rotationAngle and
@@ -123,17 +126,11 @@ struct AdaptiveArgs {
     double startx, starty, dx, dy, dt, r2;
 };
 
-/* Waves per SIMD the stream kernel is compiled for; it sets the register
- * budget (256 VGPRs at 2, 168 at 3). */
-#ifndef RT_ADAPTIVE_WAVES
-#define RT_ADAPTIVE_WAVES 2
-#endif
-
 /* NF > 0: number of Fourier coefficients known at compile time (held in
  * registers); NF == 0: a.nF coefficients read from memory per direction.
  * IX: offset type into the state (uint32_t when N * D < 2^32). */
 template <typename T, bool CURVED, int NF, typename IX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT_ADAPTIVE_WAVES))) void k_stream_adaptive(const T *__restrict__ I, T *__restrict__ Inew, AdaptiveArgs a)
+__global__ __launch_bounds__(256) void k_stream_adaptive(const T *__restrict__ I, T *__restrict__ Inew, AdaptiveArgs a)
 {
     /* the stream is continuous in every intermediate it rounds, so
      * contracting a*b+c into one FMA moves its result by a few ulp only */
