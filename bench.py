@@ -217,6 +217,32 @@ def batch_leg(rt, steps=200, n=64):
     return out
 
 
+def configs_leg(rt):
+    """The other single-problem configs at their full step counts from I = 0:
+    FP32-mixed and FP64 throughput, and FP32-mixed against FP64 (rel-L2 of I
+    and of the moments E, Fx, Fy of the final state)."""
+    out = {}
+    for name in ("minkowski_100sq_d20", "kerrschild_200sq_d32", "schwarzschild_256sq_d32"):
+        spec = rt.workloads.WORKLOADS[name]
+        K, D = spec["steps"], spec["nDir"]
+        w, phi, cx, cy = rt.build_stencil(D, tables=False)
+        res, finals = {"steps": K, "nDir": D}, {}
+        for prec, tag in ((rt.FP32_MIXED, "fp32_mixed"), (rt.FP64, "fp64")):
+            ctx, meta = rt.workloads.make_context(name, prec)
+            gs = meta["grid"]
+            act = int(np.count_nonzero(_active(gs, meta["excision_r2"])))
+            ms = ctx.time_steps(K)
+            res[tag] = {"value": act * D * K / (ms * 1e-3), "ms_per_step": ms / K, "fused": ctx.uses_fused()}
+            I = ctx.download_I().reshape(-1, D)
+            finals[tag] = (I, I @ w, I @ (w * cx), I @ (w * cy))
+            ctx.close()
+        a, b = finals["fp32_mixed"], finals["fp64"]
+        res["rel_l2_fp32_vs_fp64"] = {k: rel_l2(x, y) for k, x, y in zip(("I", "E", "Fx", "Fy"), a, b)}
+        res["fp32_over_fp64"] = res["fp32_mixed"]["value"] / res["fp64"]["value"]
+        out[name] = res
+    return out
+
+
 def adaptive_cpu_baseline(rt, name, budget_s=8.0, max_steps=10):
     """The oracle's -O3 -ffast-math OpenMP build running the adaptive workload
     from I = 0 on the host cores: one warm-up step, then as many steps as fit
@@ -300,6 +326,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-problems leg")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-streaming leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other single-problem configs")
     ap.add_argument("--only-adaptive", action="store_true", help="run only the adaptive-streaming leg")
     ap.add_argument("--ref-steps", type=int, default=200,
                     help="steps of the FP64 reference-order comparison leg (0 skips it)")
@@ -438,6 +465,9 @@ def main():
     if not args.no_adaptive:
         print("[bench] adaptive-streaming leg", file=sys.stderr, flush=True)
         result["adaptive"] = adaptive_leg(rt, cpu=not args.no_cpu)
+    if not args.no_configs:
+        print("[bench] other configs leg", file=sys.stderr, flush=True)
+        result["configs"] = configs_leg(rt)
     if not args.no_cpu:
         print("[bench] CPU baseline leg", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(rt, WORKLOAD)
