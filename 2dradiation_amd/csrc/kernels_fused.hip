@@ -72,22 +72,60 @@ struct KP {
  * 5 rows of one column at constant offsets, never wrapped; lanes
  * (= columns) hit consecutive banks.  Rows and FP64 coefficients arrive by
  * direct global-to-LDS loads, without passing through registers. */
-template <typename T, int D, bool CURVED>
+template <typename T, int D, bool CURVED, int NW>
 struct Smem {
     static constexpr int RW = 66;
     T I[3][RT_RING_ROWS(D)][RW];
     T C[CURVED ? 25 : 1][64];                    /* coefficient series (d_coefT form) */
     unsigned long long EX[CURVED ? 64 : 1];      /* excised-departure bits */
     T F[RT_CELL_FIELDS][64];                     /* per-cell fields */
-    double X[RT_FUSED_WAVES][6][64];
+    double X[NW][6][64];                         /* per-wavefront moment partials */
     float wrow[10][12] __attribute__((aligned(16)));
 };
+
+/* Eight-wavefront workgroups (D/8 directions each) for FP32 with 32 or 48
+ * directions.  The launch takes them when the 4-wavefront workgroups (at
+ * least 4 rows each) would fill at most half of one round of resident
+ * workgroups, i.e. for small grids: twice the wavefronts per workgroup then
+ * hide the gather latency.  On grids that fill the device the 4-wavefront
+ * form is faster, because every wavefront repeats the per-row collide.  Both
+ * forms give the same bits (wave_sum).  The recording variant, whose
+ * per-wavefront clamp bits the adjoint reads, always runs 4. */
+template <typename T, int D>
+constexpr bool has_wide()
+{
+    return sizeof(T) == 4 && (D == 32 || D == 48);
+}
+template <typename T, int D, bool REC>
+constexpr bool wide_ok()
+{
+    return has_wide<T, D>() && !REC;
+}
+
+/* Sum of the NW wavefront partials of moment q of this lane's cell.  Four
+ * wavefronts: ((X0 + X1) + X2) + X3.  Eight (each half of a 4-wavefront
+ * partial): the pairs first, then the same order over the pairs. */
+template <int NW>
+__device__ inline double wave_sum(const double (*X)[6][64], int q, int lane)
+{
+    if constexpr (NW == 8) {
+        double s = X[0][q][lane] + X[1][q][lane];
+#pragma unroll
+        for (int w = 2; w < NW; w += 2) s += X[w][q][lane] + X[w + 1][q][lane];
+        return s;
+    } else {
+        double s = X[0][q][lane];
+#pragma unroll
+        for (int w = 1; w < NW; w++) s += X[w][q][lane];
+        return s;
+    }
+}
 
 /* Reference lambda cycle for rows that contain a moving cell
  * (src/Radiation.cpp:661-722), with a workgroup moment exchange per cycle.
  * Every wavefront evaluates the same per-cell decisions, so the trip count is
  * uniform across the workgroup. */
-template <typename T, int DW>
+template <typename T, int DW, int NW>
 __device__ inline void collide_moving(T (&sv)[DW], double (&mOut)[6], int &cycles, const double (&mI)[6],
                                       bool active, double eta, double k0, double k1, double kA, double u1,
                                       double u2, double adt, int wv, int lane, CDirTables *dtab,
@@ -105,7 +143,9 @@ __device__ inline void collide_moving(T (&sv)[DW], double (&mOut)[6], int &cycle
     while (true) {
         const bool iter = active && fabs(diff) > 1e-4 && cycles < 100;
         if (!__any(iter)) break;
-        double part[6] = {0, 0, 0, 0, 0, 0};
+        /* two chains per 4-wavefront partial, as in the stream */
+        constexpr int NCH = (has_wide<T, DW * NW>() && NW == 4) ? 2 : 1;
+        double part[NCH][6] = {};
         if (iter) {
             cycles++;
             const double uF = u1 * prev[1] + u2 * prev[2];
@@ -128,19 +168,20 @@ __device__ inline void collide_moving(T (&sv)[DW], double (&mOut)[6], int &cycle
                 const double M = k0 * fE + 3.0 * k1 * nF;
                 const double v = ((double)sv[k] + adt * (eta + M) / (A * A)) / (1.0 + adt * (kA + k0) * A);
                 fin[k] = (T)v;
+                const int ch = (NCH == 2 && k >= DW / 2) ? 1 : 0;
 #pragma unroll
-                for (int q = 0; q < 6; q++) part[q] += dtab->w6[d][q] * v;
+                for (int q = 0; q < 6; q++) part[ch][q] += dtab->w6[d][q] * v;
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 6; q++) X[wv][q][lane] = part[q];
+        for (int q = 0; q < 6; q++) X[wv][q][lane] = NCH == 2 ? part[0][q] + part[NCH - 1][q] : part[0][q];
         __syncthreads();
         if (iter) {
             double ss = 0;
 #pragma unroll
             for (int q = 0; q < 6; q++) {
-                const double nw = ((X[0][q][lane] + X[1][q][lane]) + X[2][q][lane]) + X[3][q][lane];
+                const double nw = wave_sum<NW>(X, q, lane);
                 const double r = (prev[q] - nw) / nw;
                 ss += r * r;
                 prev[q] = nw;
@@ -154,11 +195,15 @@ __device__ inline void collide_moving(T (&sv)[DW], double (&mOut)[6], int &cycle
     for (int k = 0; k < DW; k++) sv[k] = fin[k];
 }
 
-/* two workgroups per CU where their LDS fits in 160 KB */
-template <typename T, int D, bool CURVED>
-constexpr int fused_min_blocks()
+/* Wavefronts per SIMD the register allocation must allow: as many
+ * workgroups per CU as their LDS allows in 160 KB (at most 3), each with
+ * NW/4 wavefronts on every SIMD, at most 4 in all. */
+template <typename T, int D, bool CURVED, int NW>
+constexpr int fused_waves_per_simd()
 {
-    return sizeof(Smem<T, D, CURVED>) <= 80 * 1024 ? 2 : 1;
+    constexpr int by_lds = (int)((160 * 1024) / sizeof(Smem<T, D, CURVED, NW>));
+    constexpr int wgs = by_lds < 1 ? 1 : (by_lds > 3 ? 3 : by_lds);
+    return wgs * NW / 4 > 4 ? 4 : wgs * NW / 4;
 }
 
 /* XCD-aware tile order: consecutive workgroups are dealt round-robin over
@@ -169,8 +214,8 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks)
     return (nblocks & 7) == 0 ? (bid & 7) * (nblocks >> 3) + (bid >> 3) : bid;
 }
 
-template <typename T, int D, bool CURVED, bool REC>
-__global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fused(KP<T> p)
+template <typename T, int D, bool CURVED, bool REC, int NW>
+__global__ __launch_bounds__(64 * NW, (fused_waves_per_simd<T, D, CURVED, NW>())) void k_fused(KP<T> p)
 {
     const int tile = xcd_tile(blockIdx.x, p.nblocks);
     const T *Iin = p.Iin;
@@ -183,10 +228,11 @@ __global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fus
  * problem table holds each problem's step with `parity` 0; odd parity swaps
  * the two state buffers. */
 template <typename T, int D, bool CURVED>
-__global__ __launch_bounds__(256, (fused_min_blocks<T, D, CURVED>())) void k_fused_batch(
+__global__ __launch_bounds__(64 * RT_FUSED_WAVES, (fused_waves_per_simd<T, D, CURVED, RT_FUSED_WAVES>())) void k_fused_batch(
     const KP<T> *__restrict__ probs, const int *__restrict__ start, int nprob, int parity)
 {
     constexpr bool REC = false;
+    constexpr int NW = RT_FUSED_WAVES;
     int i = 0;
     while (i + 1 < nprob && start[i + 1] <= (int)blockIdx.x) i++;
     i = __builtin_amdgcn_readfirstlane(i);
@@ -242,20 +288,37 @@ KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, con
     return p;
 }
 
+/* Resident workgroups per CU of k_fused<T, D, CURVED, REC, NW>. */
+template <typename T, int D, bool CURVED, bool REC, int NW>
+int fused_per_cu()
+{
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<T, D, CURVED, REC, NW>, 64 * NW, 0) != hipSuccess ||
+            n < 1)
+            n = 1;
+        per_cu = n;
+    }
+    return per_cu;
+}
+
 template <typename T, int D, bool CURVED, bool REC>
 int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments, const RecordPtrs *rec)
 {
     /* one round of resident workgroups: each walks a long chunk of rows,
      * which amortises the two halo rows it stages first */
-    static int per_cu = 0;
-    if (per_cu == 0) {
-        int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<T, D, CURVED, REC>, 256, 0) != hipSuccess || n < 1)
-            n = 1;
-        per_cu = n;
+    const int round4 = fused_per_cu<T, D, CURVED, REC, RT_FUSED_WAVES>() * c->ncu;
+    KP<T> p = make_kp<T>(c, in, out, want_moments, rec, 0, round4);
+    if constexpr (wide_ok<T, D, REC>()) {
+        if (2 * p.nblocks <= round4) {
+            KP<T> p8 = make_kp<T>(c, in, out, want_moments, rec, 0, fused_per_cu<T, D, CURVED, REC, 8>() * c->ncu);
+            k_fused<T, D, CURVED, REC, 8><<<p8.nblocks, 512, 0, c->stream>>>(p8);
+            RT_HIP(c, hipGetLastError());
+            return RT_OK;
+        }
     }
-    KP<T> p = make_kp<T>(c, in, out, want_moments, rec, 0, per_cu * c->ncu);
-    k_fused<T, D, CURVED, REC><<<p.nblocks, 256, 0, c->stream>>>(p);
+    k_fused<T, D, CURVED, REC, RT_FUSED_WAVES><<<p.nblocks, 64 * RT_FUSED_WAVES, 0, c->stream>>>(p);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }
@@ -412,12 +475,12 @@ int launch_group_t(rt_batch *b, const rt_batch::Group &g, int parity)
     const KP<T> *kp = (const KP<T> *)g.d_kp;
     const int n = (int)g.members.size();
     switch (g.D) {
-    case 16: k_fused_batch<T, 16, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
-    case 20: k_fused_batch<T, 20, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
-    case 24: k_fused_batch<T, 24, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
-    case 32: k_fused_batch<T, 32, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
-    case 48: k_fused_batch<T, 48, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
-    case 64: k_fused_batch<T, 64, CURVED><<<g.nblocks, 256, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    case 16: k_fused_batch<T, 16, CURVED><<<g.nblocks, 64 * RT_FUSED_WAVES, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    case 20: k_fused_batch<T, 20, CURVED><<<g.nblocks, 64 * RT_FUSED_WAVES, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    case 24: k_fused_batch<T, 24, CURVED><<<g.nblocks, 64 * RT_FUSED_WAVES, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    case 32: k_fused_batch<T, 32, CURVED><<<g.nblocks, 64 * RT_FUSED_WAVES, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    case 48: k_fused_batch<T, 48, CURVED><<<g.nblocks, 64 * RT_FUSED_WAVES, 0, b->stream>>>(kp, g.d_start, n, parity); break;
+    case 64: k_fused_batch<T, 64, CURVED><<<g.nblocks, 64 * RT_FUSED_WAVES, 0, b->stream>>>(kp, g.d_start, n, parity); break;
     default: return bfail(b, RT_ERR_UNSUPPORTED, "batch: unsupported nDir");
     }
     BHIP(b, hipGetLastError());
