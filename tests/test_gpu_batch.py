@@ -12,7 +12,11 @@ SPECS = [  # size, nDir, spacetime, precision
     (130, 24, "minkowski", "fp32"),
     (52, 32, "schwarzschild", "fp32"),
     (66, 48, "schwarzschild", "fp64"),
+    (58, 48, "schwarzschild", "fp32"),
 ]
+# FP32 members at 32 and 48 directions are small enough that stepping them
+# alone takes the 8-wavefront form of the fused kernel, while the batch runs
+# 4 wavefronts: the bitwise comparisons below cover that both agree.
 
 
 def members(rt):
@@ -34,7 +38,7 @@ def test_batch_matches_individual_steps(rt):
     A = members(rt)
     B = members(rt)
     b = rt.Batch(A)
-    assert b.groups() == 6
+    assert b.groups() == 7
     for n in (7, 4):  # odd then even: both parities of the graph
         b.step(n)
         for c in B:
