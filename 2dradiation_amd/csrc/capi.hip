@@ -344,6 +344,12 @@ int rt_set_stencil(rt_ctx *c, const double *w, const double *phi, const double *
             if (Nc[k] < 0 || Nc[k] >= D) return rt_fail(c, RT_ERR_ARG, "interpolation table index out of range");
         if (!all_finite(Wc, 4L * nGrid)) return rt_fail(c, RT_ERR_NONFINITE, "interpolation weights contain NaN/Inf");
     }
+    for (int q = 0; q < 10; q++) c->stencil_sums[q] = 0;
+    for (int d = 0; d < D; d++) {
+        const double x = cx[d], y = cy[d];
+        const double t[10] = {1, x, y, x * x, x * y, y * y, x * x * x, x * x * y, x * y * y, y * y * y};
+        for (int q = 0; q < 10; q++) c->stencil_sums[q] += w[d] * t[q];
+    }
     RT_HIP(c, hipMemcpyAsync(c->d_w, w, D * sizeof(double), hipMemcpyHostToDevice, c->stream));
     RT_HIP(c, hipMemcpyAsync(c->d_phi, phi, D * sizeof(double), hipMemcpyHostToDevice, c->stream));
     RT_HIP(c, hipMemcpyAsync(c->d_cx, cx, D * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -768,7 +774,8 @@ int rt_step(rt_ctx *c, int32_t nSteps, int32_t want_moments)
             c->cur = 1 - c->cur;
             c->rcur = 1 - c->rcur;
             TRY(launch_ref_moments(c, c->d_I[c->cur]));
-            TRY(launch_ref_collide(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+            if (c->fp32) TRY(launch_collide_adaptive(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+            else TRY(launch_ref_collide(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
             c->cur = 1 - c->cur;
         }
     } else {

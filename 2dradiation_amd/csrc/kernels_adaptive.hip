@@ -25,6 +25,7 @@
  * direction are coalesced over the 64 cells.
  */
 #include "fused_common.h"
+#include "ref_collide.h"
 
 namespace {
 
@@ -492,6 +493,170 @@ int launch_stream_adaptive(rt_ctx *c, const void *in, void *out)
         if (curved) launch_stream_t<double, true>(c, in, out, a, blocks);
         else launch_stream_t<double, false>(c, in, out, a, blocks);
     }
+    RT_HIP(c, hipGetLastError());
+    return RT_OK;
+}
+
+/* ---- Collide of a rotated stencil in moment form (FP32-mixed) ----
+ * For a cell at rest (u = 0 in the tetrad frame) every cycle of the
+ * reference's lambda iteration (src/Radiation.cpp:661-722) is affine in the
+ * moments: with q = 1 / (1 + alpha dt (kA + k0)), b = alpha dt q (eta + k0 E)
+ * and c = 3 k1 alpha dt q,
+ *   v_d = q I_d + b + c n_d.F,
+ * so the moments of v are
+ *   m'_X = q mI_X + b S0_X + c (Fx S1_X + Fy S2_X)
+ * with mI the moments of the streamed intensities and S0_X = sum_d w_d X_d,
+ * S1_X = sum_d w_d n_d,x X_d, S2_X = sum_d w_d n_d,y X_d over the cell's
+ * rotated directions n_d = R(theta) C(d): power sums of the stencil up to
+ * third order, rotated from the unrotated ones as tensors.  This is the
+ * fused kernel's moment-space iteration with per-cell stencil sums: the same
+ * stopping rule on the moments, then one pass over the directions,
+ *   Inew_d = I_d + ((q - 1) I_d + b + C(d).G),  G = c R(theta)^T F,
+ * in the storage precision.  A workgroup is 64 cells x 4 wavefronts: wave 0
+ * iterates (or, for a moving cell, runs the reference cycle,
+ * ref_collide_cell, over all directions) and hands q - 1, b and G to the
+ * waves through LDS; wave g then writes directions g, g+4, ... of the cells
+ * at rest, 8 loads in flight at a time. */
+struct StencilSums {
+    double s[10];
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_collide_adaptive(const T *__restrict__ I, T *__restrict__ Inew,
+                                                          double *__restrict__ mom, int32_t *__restrict__ count,
+                                                          const double *__restrict__ f, const double *__restrict__ alpha,
+                                                          const double *__restrict__ t11, const double *__restrict__ t12,
+                                                          const double *__restrict__ t21, const double *__restrict__ t22,
+                                                          const uint8_t *__restrict__ mask, const double *__restrict__ w,
+                                                          const double *__restrict__ cx, const double *__restrict__ cy,
+                                                          int N, int D, double dt, unsigned long long *stats,
+                                                          const double *__restrict__ rot, StencilSums ss)
+{
+    __shared__ T s_qm1[64], s_b[64], s_gx[64], s_gy[64];
+    __shared__ int s_rest[64];
+    const int lane = threadIdx.x & 63;
+    const int g = threadIdx.x >> 6;
+    /* consecutive 64-cell chunks go to one XCD (workgroups are dealt
+     * round-robin over the 8 XCDs): a direction row of 64 cells is not
+     * 128-byte aligned, and its partial cache lines at both ends are then
+     * written back by one L2 instead of two */
+    const int nb = gridDim.x;
+    const int chunk = (nb & 7) == 0 ? (int)(blockIdx.x & 7) * (nb >> 3) + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+    const long ij = (long)chunk * 64 + lane;
+    if (g == 0) {
+        unsigned long long cyc = 0;
+        int rest = 0;
+        if (ij < N) {
+            if (!(mask[ij] & MASK_ACTIVE)) {
+                count[ij] = 0;
+            } else {
+                const double ux = f[4L * N + ij], uy = f[5L * N + ij];
+                const double u1 = t11[ij] * ux + t12[ij] * uy;
+                const double u2 = t21[ij] * ux + t22[ij] * uy;
+                int cycles = 0;
+                if (u1 == 0.0 && u2 == 0.0) {
+                    rest = 1;
+                    const double eta = f[ij], k0 = f[(long)N + ij], k1 = f[2L * N + ij], kA = f[3L * N + ij];
+                    const double adt = alpha[ij] * dt;
+                    const double q = 1.0 / (1.0 + adt * (kA + k0));
+                    const double cc = 3.0 * k1 * adt * q;
+                    const double rs = rtm::my_sin(rot[ij]), rc = rtm::my_cos(rot[ij]);
+                    /* rotated power sums */
+                    const double *u = ss.s;
+                    const double c2 = rc * rc, s2 = rs * rs, cs = rc * rs;
+                    const double a = rc * u[1] - rs * u[2], b1 = rs * u[1] + rc * u[2];
+                    const double txx = c2 * u[3] - 2.0 * cs * u[4] + s2 * u[5];
+                    const double txy = cs * u[3] + (c2 - s2) * u[4] - cs * u[5];
+                    const double tyy = s2 * u[3] + 2.0 * cs * u[4] + c2 * u[5];
+                    const double uxxx = rc * c2 * u[6] - 3.0 * c2 * rs * u[7] + 3.0 * rc * s2 * u[8] - rs * s2 * u[9];
+                    const double uxxy = c2 * rs * u[6] + (rc * c2 - 2.0 * rc * s2) * u[7] +
+                                        (rs * s2 - 2.0 * c2 * rs) * u[8] + rc * s2 * u[9];
+                    const double uxyy = rc * s2 * u[6] + (2.0 * c2 * rs - rs * s2) * u[7] +
+                                        (rc * c2 - 2.0 * rc * s2) * u[8] - c2 * rs * u[9];
+                    const double uyyy = rs * s2 * u[6] + 3.0 * s2 * rc * u[7] + 3.0 * rs * c2 * u[8] + rc * c2 * u[9];
+                    const double S0[6] = {u[0], a, b1, txx, txy, tyy};
+                    const double S1[6] = {a, txx, txy, uxxx, uxxy, uxyy};
+                    const double S2[6] = {b1, txy, tyy, uxxy, uxyy, uyyy};
+                    double mI[6], prev[6];
+#pragma unroll
+                    for (int X = 0; X < 6; X++) prev[X] = mI[X] = mom[(long)X * N + ij];
+                    /* stop when sqrt(sum ((prev - new) / new)^2) <= 1e-4; new == 0
+                     * keeps the IEEE quotient (inf, or NaN for 0/0, which stops) */
+                    double b = 0, fFx = 0, fFy = 0, sq = 1;
+                    while (sq > 1e-8 && cycles < 100) {
+                        cycles++;
+                        b = adt * q * (eta + k0 * prev[0]);
+                        fFx = prev[1];
+                        fFy = prev[2];
+                        double acc = 0;
+#pragma unroll
+                        for (int X = 0; X < 6; X++) {
+                            const double nw = q * mI[X] + b * S0[X] + cc * (fFx * S1[X] + fFy * S2[X]);
+                            const double r = (prev[X] - nw) / nw;
+                            acc += r * r;
+                            prev[X] = nw;
+                        }
+                        sq = acc;
+                    }
+#pragma unroll
+                    for (int X = 0; X < 6; X++) mom[(long)X * N + ij] = prev[X];
+                    s_qm1[lane] = (T)(q - 1.0);
+                    s_b[lane] = (T)b;
+                    s_gx[lane] = (T)(cc * (rc * fFx + rs * fFy));
+                    s_gy[lane] = (T)(cc * (rc * fFy - rs * fFx));
+                } else {
+                    cycles = ref_collide_cell<T>(ij, I, Inew, mom, f, alpha, t11, t12, t21, t22, w, cx, cy, N, D, dt, rot);
+                }
+                count[ij] = cycles;
+                cyc = (unsigned long long)cycles;
+            }
+        }
+        s_rest[lane] = rest;
+        /* cycle statistics: one atomic pair per workgroup */
+        unsigned long long sum = cyc, mx = cyc;
+        for (int off = 32; off > 0; off >>= 1) {
+            sum += __shfl_xor(sum, off);
+            const unsigned long long o = __shfl_xor(mx, off);
+            mx = mx > o ? mx : o;
+        }
+        if (lane == 0 && mx > 0) {
+            atomicAdd(&stats[0], sum);
+            atomicMax(&stats[1], mx);
+        }
+    }
+    __syncthreads();
+    if (!s_rest[lane]) return;
+    const T qm1 = s_qm1[lane], bT = s_b[lane], gx = s_gx[lane], gy = s_gy[lane];
+    for (int d0 = g; d0 < D; d0 += 32) {
+        T v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (d0 + 4 * k < D) v[k] = I[(long)(d0 + 4 * k) * N + ij];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int d = d0 + 4 * k;
+            if (d < D) {
+                const T t = (T)cx[d] * gx + (T)cy[d] * gy;
+                Inew[(long)d * N + ij] = v[k] + (qm1 * v[k] + (bT + t));
+            }
+        }
+    }
+}
+
+int launch_collide_adaptive(rt_ctx *c, const void *in, void *out)
+{
+    StencilSums ss;
+    for (int q = 0; q < 10; q++) ss.s[q] = c->stencil_sums[q];
+    const double *rot = c->d_rot[c->rcur];
+    const unsigned blocks = (nblk(c->N, 64) + 7) / 8 * 8;
+    if (c->fp32)
+        k_collide_adaptive<float><<<blocks, 256, 0, c->stream>>>(
+            (const float *)in, (float *)out, c->d_mom, c->d_count, c->d_f64, c->d_alpha, c->d_t11, c->d_t12, c->d_t21,
+            c->d_t22, c->d_mask, c->d_w, c->d_cx, c->d_cy, c->N, c->D, c->desc.dt, c->d_stats, rot, ss);
+    else
+        k_collide_adaptive<double><<<blocks, 256, 0, c->stream>>>(
+            (const double *)in, (double *)out, c->d_mom, c->d_count, c->d_f64, c->d_alpha, c->d_t11, c->d_t12,
+            c->d_t21, c->d_t22, c->d_mask, c->d_w, c->d_cx, c->d_cy, c->N, c->D, c->desc.dt, c->d_stats, rot, ss);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }

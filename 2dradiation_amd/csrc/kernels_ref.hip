@@ -8,6 +8,7 @@
  */
 #include <cstring>
 #include "rt_internal.h"
+#include "ref_collide.h"
 
 namespace {
 
@@ -243,20 +244,30 @@ __global__ void k_ref_moments(const T *__restrict__ I, double *__restrict__ mom,
     if (ij >= N) return;
     double e = 0, fx = 0, fy = 0, pxx = 0, pxy = 0, pyy = 0;
     const double rs = rot ? rtm::my_sin(rot[ij]) : 0.0, rc = rot ? rtm::my_cos(rot[ij]) : 1.0;
-    for (int d = 0; d < D; d++) {
-        if (w[d] == 0.0) continue;
-        double n1 = cx[d], n2 = cy[d];
-        if (rot) {
-            n1 = cx[d] * rc - cy[d] * rs;
-            n2 = cx[d] * rs + cy[d] * rc;
+    /* 8 directions' loads are issued before their sums (same summation
+     * order as one direction at a time) */
+    for (int d0 = 0; d0 < D; d0 += 8) {
+        T v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (d0 + k < D) v[k] = I[(long)(d0 + k) * N + ij];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int d = d0 + k;
+            if (d >= D || w[d] == 0.0) continue;
+            double n1 = cx[d], n2 = cy[d];
+            if (rot) {
+                n1 = cx[d] * rc - cy[d] * rs;
+                n2 = cx[d] * rs + cy[d] * rc;
+            }
+            double c = w[d] * (double)v[k];
+            e += c;
+            fx += c * n1;
+            fy += c * n2;
+            pxx += c * n1 * n1;
+            pxy += c * n1 * n2;
+            pyy += c * n2 * n2;
         }
-        double c = w[d] * (double)I[(long)d * N + ij];
-        e += c;
-        fx += c * n1;
-        fy += c * n2;
-        pxx += c * n1 * n1;
-        pxy += c * n1 * n2;
-        pyy += c * n2 * n2;
     }
     mom[ij] = e;
     mom[(long)N + ij] = fx;
@@ -283,93 +294,7 @@ __global__ void k_ref_collide(const T *__restrict__ I, T *__restrict__ Inew, dou
         count[ij] = 0;
         return;
     }
-    double *E = mom, *Fx = mom + N, *Fy = mom + 2L * N, *Pxx = mom + 3L * N, *Pxy = mom + 4L * N,
-           *Pyy = mom + 5L * N;
-    const double eta = f[ij], k0 = f[(long)N + ij], k1 = f[2L * N + ij], kA = f[3L * N + ij];
-    double ux = f[4L * N + ij], uy = f[5L * N + ij];
-    double u1 = t11[ij] * ux + t12[ij] * uy;
-    double u2 = t21[ij] * ux + t22[ij] * uy;
-    double al = alpha[ij];
-    double uu = u1 * u1 + u2 * u2;
-    double lorentz = 1.0 / sqrt(1.0 - uu);
-    /* nIF = RotationMatrix(rotationAngle[ij]) * C(d) (:691) */
-    const double rs = rot ? rtm::my_sin(rot[ij]) : 0.0, rc = rot ? rtm::my_cos(rot[ij]) : 1.0;
-    int cycles = 0;
-    double diff = 1;
-    while (fabs(diff) > 1e-4 && (cycles < 100)) {
-        cycles++;
-        double pE = E[ij], pFx = Fx[ij], pFy = Fy[ij], pPxx = Pxx[ij], pPxy = Pxy[ij], pPyy = Pyy[ij];
-        double uF = u1 * pFx + u2 * pFy;
-        double uxP = u1 * pPxx + u2 * pPxy;
-        double uyP = u1 * pPxy + u2 * pPyy;
-        double uuP = u1 * uxP + u2 * uyP;
-        double fE = lorentz * lorentz * (pE - 2.0 * uF + uuP);
-        double fFx = lorentz * lorentz *
-                     (pFx - lorentz * pE * u1 - uxP +
-                      u1 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
-        double fFy = lorentz * lorentz *
-                     (pFy - lorentz * pE * u2 - uyP +
-                      u2 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
-        double e = 0, fx = 0, fy = 0, pxx = 0, pxy = 0, pyy = 0;
-        if (u1 == 0.0 && u2 == 0.0) {
-            /* Fluid at rest: lorentz = 1, u.n = 0, A = 1, fac = 1 and
-             * nFF = n exactly, so dropping the factors and divisions by 1
-             * below leaves every rounded value of the general form unchanged
-             * (up to the sign of zeros). */
-            const double adt = al * dt, den = 1.0 + al * dt * (kA + k0);
-            for (int d = 0; d < D; d++) {
-                const long idx = (long)d * N + ij;
-                double n1 = cx[d], n2 = cy[d];
-                if (rot) {
-                    n1 = cx[d] * rc - cy[d] * rs;
-                    n2 = cx[d] * rs + cy[d] * rc;
-                }
-                const double nF = n1 * fFx + n2 * fFy;
-                const double M = k0 * fE + 3.0 * k1 * nF;
-                const double v = ((double)I[idx] + adt * (eta + M)) / den;
-                Inew[idx] = (T)v;
-                const double c = w[d] * v;
-                e += c;
-                fx += c * n1;
-                fy += c * n2;
-                pxx += c * n1 * n1;
-                pxy += c * n1 * n2;
-                pyy += c * n2 * n2;
-            }
-        } else
-        for (int d = 0; d < D; d++) {
-            long idx = (long)d * N + ij;
-            double n1 = cx[d], n2 = cy[d];
-            if (rot) {
-                n1 = cx[d] * rc - cy[d] * rs;
-                n2 = cx[d] * rs + cy[d] * rc;
-            }
-            double un = u1 * n1 + u2 * n2;
-            double A = lorentz * (1.0 - un);
-            double fac = (1 - (lorentz * un) / (lorentz + 1)) * lorentz;
-            double nf1 = (n1 - fac * u1) / A, nf2 = (n2 - fac * u2) / A;
-            double nF = nf1 * fFx + nf2 * fFy;
-            double M = k0 * fE + 3.0 * k1 * nF;
-            double v = ((double)I[idx] + al * dt * (eta + M) / (A * A)) / (1.0 + al * dt * (kA + k0) * A);
-            Inew[idx] = (T)v;
-            double c = w[d] * v;
-            e += c;
-            fx += c * n1;
-            fy += c * n2;
-            pxx += c * n1 * n1;
-            pxy += c * n1 * n2;
-            pyy += c * n2 * n2;
-        }
-        E[ij] = e;
-        Fx[ij] = fx;
-        Fy[ij] = fy;
-        Pxx[ij] = pxx;
-        Pxy[ij] = pxy;
-        Pyy[ij] = pyy;
-        double dE = (pE - e) / e, dFx = (pFx - fx) / fx, dFy = (pFy - fy) / fy;
-        double dPxx = (pPxx - pxx) / pxx, dPxy = (pPxy - pxy) / pxy, dPyy = (pPyy - pyy) / pyy;
-        diff = sqrt(dE * dE + dFx * dFx + dFy * dFy + dPxx * dPxx + dPxy * dPxy + dPyy * dPyy);
-    }
+    const int cycles = ref_collide_cell<T>(ij, I, Inew, mom, f, alpha, t11, t12, t21, t22, w, cx, cy, N, D, dt, rot);
     count[ij] = cycles;
     atomicAdd(&stats[0], (unsigned long long)cycles);
     atomicMax(&stats[1], (unsigned long long)cycles);

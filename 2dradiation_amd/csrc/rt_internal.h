@@ -116,6 +116,10 @@ struct rt_ctx {
     float *d_rtrigf = nullptr;
     float *d_Wcf = nullptr;
     bool coefT_valid = false;
+    /* weighted power sums of the unrotated intensity stencil, for the
+     * adaptive collide's moment form: sum w, w cx, w cy, w cx^2, w cx cy,
+     * w cy^2, w cx^3, w cx^2 cy, w cx cy^2, w cy^3 */
+    double stencil_sums[10] = {};
     double *d_src_rot = nullptr;
     bool src_rot_set = false;
     int32_t *d_count = nullptr;   /* [N] per-method collide cycle counts */
@@ -168,6 +172,7 @@ int launch_rotation(rt_ctx *c, bool update);
 int launch_stream_adaptive(rt_ctx *c, const void *in, void *out);
 int launch_restore_source_rot(rt_ctx *c);
 bool adaptive_f32_ready(const rt_ctx *c);
+int launch_collide_adaptive(rt_ctx *c, const void *in, void *out);
 inline bool rt_is_adaptive(const rt_desc &d) { return d.streaming == RT_FLAT_ADAPTIVE || d.streaming == RT_CURVED_ADAPTIVE; }
 inline bool rt_is_curved(const rt_desc &d) { return d.streaming == RT_CURVED_FIXED || d.streaming == RT_CURVED_ADAPTIVE; }
 int launch_ref_moments(rt_ctx *c, const void *in);
