@@ -244,15 +244,17 @@ __global__ void k_ref_moments(const T *__restrict__ I, double *__restrict__ mom,
     if (ij >= N) return;
     double e = 0, fx = 0, fy = 0, pxx = 0, pxy = 0, pyy = 0;
     const double rs = rot ? rtm::my_sin(rot[ij]) : 0.0, rc = rot ? rtm::my_cos(rot[ij]) : 1.0;
-    /* 8 directions' loads are issued before their sums (same summation
-     * order as one direction at a time) */
-    for (int d0 = 0; d0 < D; d0 += 8) {
-        T v[8];
+    /* FP32: 8 directions' loads are issued before their sums (same
+     * summation order as one direction at a time; FP64 measured faster
+     * one at a time) */
+    constexpr int B = sizeof(T) == 4 ? 8 : 1;
+    for (int d0 = 0; d0 < D; d0 += B) {
+        T v[B];
 #pragma unroll
-        for (int k = 0; k < 8; k++)
+        for (int k = 0; k < B; k++)
             if (d0 + k < D) v[k] = I[(long)(d0 + k) * N + ij];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < B; k++) {
             const int d = d0 + k;
             if (d >= D || w[d] == 0.0) continue;
             double n1 = cx[d], n2 = cy[d];
