@@ -177,13 +177,20 @@ def test_adaptive_single_methods_vs_oracle(rt, O):
     ctx.close()
 
 
-@pytest.mark.parametrize("kind", [0, 2])
-def test_adaptive_fp32_mixed(rt, O, kind):
+@pytest.mark.parametrize("kind,motion", [(0, "rest"), (2, "rest"), (2, "moving"), (0, "mixed")])
+def test_adaptive_fp32_mixed(rt, O, kind, motion):
+    """FP32-mixed (moment-form collide for cells at rest, the reference cycle
+    for moving ones, both in one launch for "mixed") against FP64 (the
+    reference-order collide)."""
     n, n_order, n_ghost, steps = 28, 16, 4, 10
     N = (n + 2) ** 2
     bh = 2.1 * 2.1 if kind else -1.0
     span = 4.0 if kind else 1.0
-    fields = fields_for(N, 9, u=False)
+    fields = fields_for(N, 9, u=motion != "rest")
+    if motion == "mixed":
+        still = np.random.default_rng(5).uniform(0, 1, N) < 0.5
+        fields["ux"][still] = 0.0
+        fields["uy"][still] = 0.0
     out = {}
     for prec in (rt.FP64, rt.FP32_MIXED):
         P, ctx, gs, og = setup(rt, O, kind, n_order, n_ghost, n, prec, fields, bh, span)
