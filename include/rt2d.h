@@ -180,7 +180,9 @@ int rt_get_iteration_counts(rt_ctx *ctx, int32_t *count);
 /* nSteps full steps.  FlatFixed and CurvedFixed: stream + moments + collide
  * fused in one kernel where rt_uses_fused_path says so, else the
  * reference-order kernels; FlatAdaptive and CurvedAdaptive: rotation update,
- * adaptive stream, rotated moments and collide (reference-order kernels).
+ * adaptive stream, rotated moments and collide (reference-order kernels in
+ * FP64 contexts; FP32-mixed contexts stream in FP32 and collide in moment
+ * form, the λ iteration on the moments of each cell's rotated stencil).
  * Replaces: the loop body of Radiation::RunSimulation (src/Radiation.cpp:784-832).
  * If want_moments is non-zero the moments of the last step are kept for
  * rt_get_moments (the reference-order kernels always keep them). */
