@@ -1014,7 +1014,7 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
         TRY(dalloc(c, &c->d_seg, (size_t)every * bytes));
         TRY(dalloc(c, &c->d_rec_m, (size_t)every * 6 * N * sizeof(double)));
         TRY(dalloc(c, &c->d_rec_k, (size_t)every * N));
-        TRY(dalloc(c, &c->d_rec_mask, (size_t)every * RT_FUSED_WAVES * N * sizeof(unsigned long long)));
+        TRY(dalloc(c, &c->d_rec_mask, (size_t)every * 8 * N * sizeof(unsigned long long)));
         c->seg_cap = every;
     }
     TRY(dalloc(c, &c->d_lam[0], bytes));
@@ -1033,7 +1033,7 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
         RecordPtrs r;
         r.m = c->d_rec_m + (size_t)i * 6 * N;
         r.k = c->d_rec_k + (size_t)i * N;
-        r.mask = c->d_rec_mask + (size_t)i * RT_FUSED_WAVES * N;
+        r.mask = c->d_rec_mask + (size_t)i * 8 * N;
         return r;
     };
     /* halo and excised cells keep their values in every state: copy them

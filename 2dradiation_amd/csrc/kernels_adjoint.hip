@@ -37,6 +37,7 @@ struct AP {
     const double *rec_m;
     const uint8_t *rec_k;
     const unsigned long long *rec_mask;
+    int rec_waves; /* wavefronts of the recording launch: 4, or 8 (two words per adjoint wavefront) */
     const T *cell;
     const uint8_t *mask;
     const T *coefT;
@@ -257,7 +258,18 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
                 cV[k] = sm.C[20 + k][lane];
             }
             const uint32_t exw = (uint32_t)(sm.EX[lane] >> (wv * DW));
-            const unsigned long long mbits = active ? (p.rec_mask + (size_t)wv * N)[ij] : 0ull;
+            unsigned long long mbits = 0ull;
+            if (active) {
+                if (p.rec_waves == 8) {
+                    /* recording wavefronts 2 wv and 2 wv + 1 hold this
+                     * wavefront's first and second DW/2 directions */
+                    const unsigned long long lo = (p.rec_mask + (size_t)(2 * wv) * N)[ij];
+                    const unsigned long long hi = (p.rec_mask + (size_t)(2 * wv + 1) * N)[ij];
+                    mbits = lo | (hi << (2 * DW));
+                } else {
+                    mbits = (p.rec_mask + (size_t)wv * N)[ij];
+                }
+            }
             const T angu = (sizeof(T) == 4) ? (T)dtab->ang_uf : (T)dtab->ang_u;
             const int quarter = dtab->quarter, nGrid = dtab->nGrid;
             const T ddmax = T(nGrid) - T(1e-8);
@@ -399,6 +411,7 @@ int launch_adj(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, 
     p.rec_m = rec.m;
     p.rec_k = rec.k;
     p.rec_mask = rec.mask;
+    p.rec_waves = c->rec_waves;
     p.cell = (const T *)c->d_cell;
     p.mask = c->d_mask;
     p.coefT = (const T *)c->d_coefT;

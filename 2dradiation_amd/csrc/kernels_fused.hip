@@ -89,17 +89,13 @@ struct Smem {
  * workgroups, i.e. for small grids: twice the wavefronts per workgroup then
  * hide the gather latency.  On grids that fill the device the 4-wavefront
  * form is faster, because every wavefront repeats the per-row collide.  Both
- * forms give the same bits (wave_sum).  The recording variant, whose
- * per-wavefront clamp bits the adjoint reads, always runs 4. */
+ * forms give the same bits (wave_sum).  The recording variant notes the
+ * form it ran (rt_ctx::rec_waves): the adjoint reads its per-wavefront clamp
+ * bits accordingly. */
 template <typename T, int D>
 constexpr bool has_wide()
 {
     return sizeof(T) == 4 && (D == 32 || D == 48);
-}
-template <typename T, int D, bool REC>
-constexpr bool wide_ok()
-{
-    return has_wide<T, D>() && !REC;
 }
 
 /* Sum of the NW wavefront partials of moment q of this lane's cell.  Four
@@ -310,14 +306,16 @@ int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments, const Re
      * which amortises the two halo rows it stages first */
     const int round4 = fused_per_cu<T, D, CURVED, REC, RT_FUSED_WAVES>() * c->ncu;
     KP<T> p = make_kp<T>(c, in, out, want_moments, rec, 0, round4);
-    if constexpr (wide_ok<T, D, REC>()) {
+    if constexpr (has_wide<T, D>()) {
         if (2 * p.nblocks <= round4) {
             KP<T> p8 = make_kp<T>(c, in, out, want_moments, rec, 0, fused_per_cu<T, D, CURVED, REC, 8>() * c->ncu);
             k_fused<T, D, CURVED, REC, 8><<<p8.nblocks, 512, 0, c->stream>>>(p8);
             RT_HIP(c, hipGetLastError());
+            if (REC) c->rec_waves = 8;
             return RT_OK;
         }
     }
+    if (REC) c->rec_waves = RT_FUSED_WAVES;
     k_fused<T, D, CURVED, REC, RT_FUSED_WAVES><<<p.nblocks, 64 * RT_FUSED_WAVES, 0, c->stream>>>(p);
     RT_HIP(c, hipGetLastError());
     return RT_OK;

@@ -135,7 +135,8 @@ struct rt_ctx {
     void *d_seg = nullptr;
     double *d_rec_m = nullptr;
     uint8_t *d_rec_k = nullptr;
-    unsigned long long *d_rec_mask = nullptr;
+    unsigned long long *d_rec_mask = nullptr; /* [every][8][N]: clamp bits per recording wavefront */
+    int rec_waves = RT_FUSED_WAVES;             /* wavefronts of the last recording launch (4 or 8) */
     void *d_lam[2] = {nullptr, nullptr};
     double *d_grad[2] = {nullptr, nullptr};
     double *d_pts = nullptr; /* sample points and their gradients */
