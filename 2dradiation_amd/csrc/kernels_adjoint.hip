@@ -435,7 +435,10 @@ int launch_adj(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, 
     const int wgs = per_cu * c->ncu;
     int chunks = (wgs + strips - 1) / strips;
     int H = (rows + chunks - 1) / chunks;
-    H = H < 4 ? 4 : (H > 64 ? 64 : H);
+    /* chunks down to 2 rows: on small grids that fills a second resident
+     * workgroup per CU where the LDS allows it (FP32 256^2 x 32: 40 against
+     * 51 ms for 500 adjoint steps), at the price of more halo-row atomics */
+    H = H < 2 ? 2 : (H > 64 ? 64 : H);
     chunks = (rows + H - 1) / H;
     p.chunks = chunks;
     p.rows_per_block = H;
