@@ -84,9 +84,9 @@ struct Smem {
 };
 
 /* Eight-wavefront workgroups (D/8 directions each) for FP32 with 32 or 48
- * directions.  The launch takes them when the 4-wavefront workgroups (at
- * least 4 rows each) would fill at most half of one round of resident
- * workgroups, i.e. for small grids: twice the wavefronts per workgroup then
+ * directions.  The launch takes them when the 4-wavefront workgroups would
+ * fill at most half of one round of resident workgroups even with one-row
+ * chunks, i.e. for very small grids: twice the wavefronts per workgroup then
  * hide the gather latency.  On grids that fill the device the 4-wavefront
  * form is faster, because every wavefront repeats the per-row collide.  Both
  * forms give the same bits (wave_sum).  The recording variant notes the
@@ -240,7 +240,10 @@ __global__ __launch_bounds__(64 * RT_FUSED_WAVES, (fused_waves_per_simd<T, D, CU
 }
 
 /* Kernel parameters of one fused step of context c; H = rows per workgroup
- * (0: about `wgs` workgroups with 4 to 64 rows each). */
+ * (0: about `wgs` workgroups with 1 to 64 rows each; on small grids, chunks
+ * of 1 or 2 rows fill the device at the price of staging their halo rows
+ * again: 100^2 x 64 takes 17.7 us per step against 30.4 us with chunks of
+ * at least 4 rows, 200^2 x 32 Kerr-Schild 26.3 against 32.6 us). */
 template <typename T>
 KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, const RecordPtrs *rec, int H = 0,
               int wgs = 1024)
@@ -265,7 +268,7 @@ KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, con
     if (H <= 0) {
         const int target = (wgs + p.strips - 1) / p.strips;
         H = (rows + target - 1) / target;
-        H = H < 4 ? 4 : (H > 64 ? 64 : H);
+        H = H < 1 ? 1 : (H > 64 ? 64 : H);
     }
     const int chunks = (rows + H - 1) / H;
     p.chunks = chunks;
