@@ -672,7 +672,7 @@ int rt_batch_autotune(rt_batch *b, int32_t trial_steps, int32_t *chosen)
             return bfail(b, RT_ERR_HIP, "autotune: state copy failed");
         }
     }
-    static const int candidates[] = {0, 4, 8, 16, 32, 64};
+    static const int candidates[] = {0, 1, 2, 4, 8, 16, 32, 64};
     int gi = 0;
     for (auto &g : b->groups) {
         float best = 1e30f;
