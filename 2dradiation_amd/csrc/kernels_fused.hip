@@ -84,18 +84,22 @@ struct Smem {
 };
 
 /* Eight-wavefront workgroups (D/8 directions each) for FP32 with 32 or 48
- * directions.  The launch takes them when the 4-wavefront workgroups would
- * fill at most half of one round of resident workgroups even with one-row
- * chunks, i.e. for very small grids: twice the wavefronts per workgroup then
- * hide the gather latency.  On grids that fill the device the 4-wavefront
- * form is faster, because every wavefront repeats the per-row collide.  Both
- * forms give the same bits (wave_sum).  The recording variant notes the
- * form it ran (rt_ctx::rec_waves): the adjoint reads its per-wavefront clamp
- * bits accordingly. */
+ * directions and FP64 with 32, 48 or 64.  The launch takes them when the
+ * 4-wavefront workgroups would fill at most half of one round of resident
+ * workgroups even with one-row chunks, i.e. for very small grids: twice the
+ * wavefronts per workgroup then hide the gather latency.  On grids that
+ * fill the device the 4-wavefront form is faster, because every wavefront
+ * repeats the per-row collide, except where the LDS of the 4-wavefront form
+ * allows only one workgroup per CU (FP64 at 32 to 64 curved directions):
+ * there the 8-wavefront form doubles the wavefronts per SIMD at the same
+ * LDS and is faster on any grid (FP64 1024^2 x 64: 716 against 821 us per
+ * step).  Both forms give the same bits (wave_sum).  The recording variant
+ * notes the form it ran (rt_ctx::rec_waves): the adjoint reads its
+ * per-wavefront clamp bits accordingly. */
 template <typename T, int D>
 constexpr bool has_wide()
 {
-    return sizeof(T) == 4 && (D == 32 || D == 48);
+    return (sizeof(T) == 4 && (D == 32 || D == 48)) || (sizeof(T) == 8 && (D == 32 || D == 48 || D == 64));
 }
 
 /* Sum of the NW wavefront partials of moment q of this lane's cell.  Four
@@ -310,7 +314,8 @@ int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments, const Re
     const int round4 = fused_per_cu<T, D, CURVED, REC, RT_FUSED_WAVES>() * c->ncu;
     KP<T> p = make_kp<T>(c, in, out, want_moments, rec, 0, round4);
     if constexpr (has_wide<T, D>()) {
-        if (2 * p.nblocks <= round4) {
+        constexpr bool one_wg = fused_waves_per_simd<T, D, CURVED, RT_FUSED_WAVES>() == 1;
+        if (one_wg || 2 * p.nblocks <= round4) {
             KP<T> p8 = make_kp<T>(c, in, out, want_moments, rec, 0, fused_per_cu<T, D, CURVED, REC, 8>() * c->ncu);
             k_fused<T, D, CURVED, REC, 8><<<p8.nblocks, 512, 0, c->stream>>>(p8);
             RT_HIP(c, hipGetLastError());
