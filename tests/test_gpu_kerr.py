@@ -1,4 +1,4 @@
-"""Kerr-Schild with spin (survey row f4: a != 0) and the per-step table
+"""Kerr-Schild with spin (a != 0) and the per-step table
 rebuild (rt_rebuild_fourier_tables, Config::updateFourierHarmonics,
 src/Radiation.cpp:790-791).  The metric with spin follows
 KerrSchild::MetricFunction (src/Spacetimes.cpp:87-111) in both the library's
