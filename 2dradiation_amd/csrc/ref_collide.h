@@ -36,6 +36,9 @@ __device__ inline int ref_collide_cell(long ij, const T *__restrict__ I, T *__re
     const double rs = rot ? rtm::my_sin(rot[ij]) : 0.0, rc = rot ? rtm::my_cos(rot[ij]) : 1.0;
     int cycles = 0;
     double diff = 1;
+    /* the inputs of the last cycle: Inew is written once, after the loop,
+     * with that cycle's arithmetic */
+    double lastE = 0, lastFx = 0, lastFy = 0;
     while (fabs(diff) > 1e-4 && (cycles < 100)) {
         cycles++;
         double pE = E[ij], pFx = Fx[ij], pFy = Fy[ij], pPxx = Pxx[ij], pPxy = Pxy[ij], pPyy = Pyy[ij];
@@ -50,6 +53,9 @@ __device__ inline int ref_collide_cell(long ij, const T *__restrict__ I, T *__re
         double fFy = lorentz * lorentz *
                      (pFy - lorentz * pE * u2 - uyP +
                       u2 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
+        lastE = fE;
+        lastFx = fFx;
+        lastFy = fFy;
         double e = 0, fx = 0, fy = 0, pxx = 0, pxy = 0, pyy = 0;
         if (u1 == 0.0 && u2 == 0.0) {
             /* Fluid at rest: lorentz = 1, u.n = 0, A = 1, fac = 1 and
@@ -67,7 +73,6 @@ __device__ inline int ref_collide_cell(long ij, const T *__restrict__ I, T *__re
                 const double nF = n1 * fFx + n2 * fFy;
                 const double M = k0 * fE + 3.0 * k1 * nF;
                 const double v = ((double)I[idx] + adt * (eta + M)) / den;
-                Inew[idx] = (T)v;
                 const double c = w[d] * v;
                 e += c;
                 fx += c * n1;
@@ -91,7 +96,6 @@ __device__ inline int ref_collide_cell(long ij, const T *__restrict__ I, T *__re
             double nF = nf1 * fFx + nf2 * fFy;
             double M = k0 * fE + 3.0 * k1 * nF;
             double v = ((double)I[idx] + al * dt * (eta + M) / (A * A)) / (1.0 + al * dt * (kA + k0) * A);
-            Inew[idx] = (T)v;
             double c = w[d] * v;
             e += c;
             fx += c * n1;
@@ -109,6 +113,37 @@ __device__ inline int ref_collide_cell(long ij, const T *__restrict__ I, T *__re
         double dE = (pE - e) / e, dFx = (pFx - fx) / fx, dFy = (pFy - fy) / fy;
         double dPxx = (pPxx - pxx) / pxx, dPxy = (pPxy - pxy) / pxy, dPyy = (pPyy - pyy) / pyy;
         diff = sqrt(dE * dE + dFx * dFx + dFy * dFy + dPxx * dPxx + dPxy * dPxy + dPyy * dPyy);
+    }
+    const double fE = lastE, fFx = lastFx, fFy = lastFy;
+    if (u1 == 0.0 && u2 == 0.0) {
+        const double adt = al * dt, den = 1.0 + al * dt * (kA + k0);
+        for (int d = 0; d < D; d++) {
+            const long idx = (long)d * N + ij;
+            double n1 = cx[d], n2 = cy[d];
+            if (rot) {
+                n1 = cx[d] * rc - cy[d] * rs;
+                n2 = cx[d] * rs + cy[d] * rc;
+            }
+            const double nF = n1 * fFx + n2 * fFy;
+            const double M = k0 * fE + 3.0 * k1 * nF;
+            Inew[idx] = (T)(((double)I[idx] + adt * (eta + M)) / den);
+        }
+    } else {
+        for (int d = 0; d < D; d++) {
+            long idx = (long)d * N + ij;
+            double n1 = cx[d], n2 = cy[d];
+            if (rot) {
+                n1 = cx[d] * rc - cy[d] * rs;
+                n2 = cx[d] * rs + cy[d] * rc;
+            }
+            double un = u1 * n1 + u2 * n2;
+            double A = lorentz * (1.0 - un);
+            double fac = (1 - (lorentz * un) / (lorentz + 1)) * lorentz;
+            double nf1 = (n1 - fac * u1) / A, nf2 = (n2 - fac * u2) / A;
+            double nF = nf1 * fFx + nf2 * fFy;
+            double M = k0 * fE + 3.0 * k1 * nF;
+            Inew[idx] = (T)(((double)I[idx] + al * dt * (eta + M) / (A * A)) / (1.0 + al * dt * (kA + k0) * A));
+        }
     }
     return cycles;
 }
