@@ -339,13 +339,17 @@ class Context:
         return bool(f.value)
 
     def iteration_counts(self):
-        """Per-cell cycle counts of the last reference-order collide."""
+        """Per-cell lambda-cycle counts of the last collide that records them:
+        the reference-order collide, and every step of an FP64 context with a
+        fixed stencil (its exact collide)."""
         cnt = np.zeros(self.grid.N, dtype=np.int32)
         self._c(lib().rt_get_iteration_counts(self.h, cnt))
         return cnt
 
     def forward_checkpointed(self, n, every):
-        """n steps keeping a checkpoint every `every` steps for adjoint()."""
+        """n steps keeping a checkpoint every `every` steps for adjoint().
+        Each call replaces the checkpoints and bumps checkpoint_generation."""
+        self.checkpoint_generation = getattr(self, "checkpoint_generation", 0) + 1
         self._c(lib().rt_forward_checkpointed(self.h, n, every))
 
     def sample_outgoing(self, x, y):

@@ -44,11 +44,6 @@ bool all_finite(const double *a, size_t n)
 }
 size_t tsize(const rt_ctx *c) { return c->fp32 ? sizeof(float) : sizeof(double); }
 
-#define TRY(x)                                                                                     \
-    do {                                                                                           \
-        int _r = (x);                                                                              \
-        if (_r != RT_OK) return _r;                                                                \
-    } while (0)
 #define NEED(c, p)                                                                                 \
     do {                                                                                           \
         if (!(p)) return rt_fail((c), RT_ERR_ARG, "null pointer argument: " #p);                    \
@@ -271,6 +266,9 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
     if (d.streaming == RT_CURVED_FIXED && ((r = dalloc(c, &c->d_coefT, 5 * (size_t)d.nCoeff * N * tsize(c))) ||
                                            (r = dalloc(c, &c->d_excl, N * sizeof(unsigned long long)))))
         return fail(r);
+    /* FP64 fixed streaming: the streamed intensities between the fused
+     * stream and the exact collide */
+    if (!c->fp32 && !rt_is_adaptive(d) && (r = dalloc(c, &c->d_J, N * D * sizeof(double)))) return fail(r);
     if (rt_is_adaptive(d)) {
         if ((r = dalloc(c, &c->d_rot[0], N * sizeof(double))) || (r = dalloc(c, &c->d_rot[1], N * sizeof(double))) ||
             (r = dalloc(c, &c->d_rtrig, 4 * N * sizeof(double))))
@@ -315,7 +313,7 @@ void rt_destroy(rt_ctx *c)
     dfree(c->d_rtrigf); dfree(c->d_Wcf);
     dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
     dfree(c->d_lam[0]); dfree(c->d_lam[1]); dfree(c->d_grad[0]); dfree(c->d_grad[1]); dfree(c->d_pts); dfree(c->d_mask); dfree(c->d_f64);
-    dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_mom); dfree(c->d_count);
+    dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_J); dfree(c->d_mom); dfree(c->d_count);
     dfree(c->d_stats);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -759,7 +757,7 @@ int rt_step(rt_ctx *c, int32_t nSteps, int32_t want_moments)
         if (want_moments) TRY(launch_fill(c, c->d_mom, 6 * (size_t)c->N, 0.0));
         for (int n = 0; n < nSteps; n++) {
             TRY(launch_restore_sources(c, c->d_I[c->cur], c->scaled));
-            TRY(launch_fused_step(c, c->d_I[c->cur], c->d_I[1 - c->cur], want_moments && n == nSteps - 1));
+            TRY(launch_product_step(c, c->d_I[c->cur], c->d_I[1 - c->cur], want_moments && n == nSteps - 1));
             c->cur = 1 - c->cur;
         }
     } else if (rt_is_adaptive(c->desc)) {
@@ -934,16 +932,18 @@ int rt_forward_checkpointed(rt_ctx *c, int32_t nSteps, int32_t every)
     const int count = (nSteps + every - 1) / every;
     if (count > c->ck_cap) {
         dfree(c->d_ckpt);
+        c->ck_cap = 0;
         TRY(dalloc(c, &c->d_ckpt, (size_t)count * bytes));
         c->ck_cap = count;
     }
     TRY(prepare_fused(c));
+    c->ck_scaled = c->scaled;
     for (int s = 0; s < count; s++) {
         RT_HIP(c, hipMemcpyAsync((char *)c->d_ckpt + (size_t)s * bytes, c->d_I[c->cur], bytes, hipMemcpyDeviceToDevice,
                                  c->stream));
         const int steps = std::min(every, nSteps - s * every);
         for (int n = 0; n < steps; n++) {
-            TRY(launch_fused_step(c, c->d_I[c->cur], c->d_I[1 - c->cur], false));
+            TRY(launch_product_step(c, c->d_I[c->cur], c->d_I[1 - c->cur], false));
             c->cur = 1 - c->cur;
         }
     }
@@ -1011,6 +1011,7 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
     const int every = c->ck_every;
     if (every > c->seg_cap) {
         dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
+        c->seg_cap = 0;
         TRY(dalloc(c, &c->d_seg, (size_t)every * bytes));
         TRY(dalloc(c, &c->d_rec_m, (size_t)every * 6 * N * sizeof(double)));
         TRY(dalloc(c, &c->d_rec_k, (size_t)every * N));
@@ -1022,12 +1023,21 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
     TRY(dalloc(c, &c->d_grad[0], N * sizeof(double)));
     TRY(dalloc(c, &c->d_grad[1], N * sizeof(double)));
     TRY(upload_points(c, npts, x, y, dL_dout));
+    /* the recomputation and the seed use the checkpoints' state convention;
+     * the current state (d_I) is not touched, and its flag comes back on
+     * every return */
+    struct ScaleGuard {
+        rt_ctx *c;
+        bool saved;
+        ~ScaleGuard() { c->scaled = saved; }
+    } guard{c, c->scaled};
+    c->scaled = c->ck_scaled;
     RT_HIP(c, hipMemsetAsync(c->d_grad[0], 0, N * sizeof(double), c->stream));
     RT_HIP(c, hipMemsetAsync(c->d_grad[1], 0, N * sizeof(double), c->stream));
     int lc = 0;
     RT_HIP(c, hipMemsetAsync(c->d_lam[lc], 0, bytes, c->stream));
     /* dL/d(final state) from the sampled outputs */
-    TRY(launch_sample_adjoint(c, c->d_lam[lc], c->scaled ? 1 : 0, npts, c->d_pts, c->d_pts + npts,
+    TRY(launch_sample_adjoint(c, c->d_lam[lc], c->ck_scaled ? 1 : 0, npts, c->d_pts, c->d_pts + npts,
                               c->d_pts + 2 * npts));
     auto rec_at = [&](int i) {
         RecordPtrs r;

@@ -65,7 +65,42 @@ struct KP {
     double *rec_m;
     uint8_t *rec_k;
     unsigned long long *rec_mask;
+    /* FP64 fields eta, kappa0, kappa1, kappaA, ux, uy [6][N] and lapse [N]
+     * (zero_start_stops) */
+    const double *f64;
+    const double *alpha64;
 };
+
+/* The reference's first lambda cycle from all-zero streamed intensities
+ * (src/Radiation.cpp:661-721): with prev = 0 and u = 0 every direction gets
+ * the same v = (0 + a dt eta) / (1 + a dt (kA + k0)), the new moments are
+ * the d-ordered sums of c = w v times {1, n1, n2, n1 n1, n1 n2, n2 n2}, and
+ * (0 - new) / new is NaN, which stops the loop, wherever a sum is exactly 0.
+ * At an isotropic start the y flux is such a sum of rounding noise, exactly
+ * 0 in some cells and not in others; the moment recursion cannot see it.
+ * Evaluated in the reference's operation order, without contraction. */
+template <int D>
+__device__ __noinline__ bool zero_start_stops(const double *__restrict__ f64, const double *__restrict__ alpha64,
+                                              long ij, long N, double dt, CDirTables *dtab)
+{
+#pragma clang fp contract(off)
+    const double eta = f64[ij], k0 = f64[N + ij], kA = f64[3 * N + ij], al = alpha64[ij];
+    const double adt = al * dt, den = 1.0 + al * dt * (kA + k0);
+    const double v = (0.0 + adt * eta) / den;
+    double e = 0, fx = 0, fy = 0, pxx = 0, pxy = 0, pyy = 0;
+#pragma unroll 4
+    for (int d = 0; d < D; d++) {
+        const double n1 = dtab->cx[d], n2 = dtab->cy[d];
+        const double c = dtab->w6[d][0] * v;
+        e += c;
+        fx += c * n1;
+        fy += c * n2;
+        pxx += c * n1 * n1;
+        pxy += c * n1 * n2;
+        pyy += c * n2 * n2;
+    }
+    return e == 0.0 || fx == 0.0 || fy == 0.0 || pxx == 0.0 || pxy == 0.0 || pyy == 0.0;
+}
 
 /* LDS of one workgroup.  Intensity rows are SoA [direction][66 columns]
  * holding directions -2..D+3 (mod D), so the 5 angular taps of a corner are
@@ -214,7 +249,7 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks)
     return (nblocks & 7) == 0 ? (bid & 7) * (nblocks >> 3) + (bid >> 3) : bid;
 }
 
-template <typename T, int D, bool CURVED, bool REC, int NW>
+template <typename T, int D, bool CURVED, bool REC, int NW, bool SONLY>
 __global__ __launch_bounds__(64 * NW, (fused_waves_per_simd<T, D, CURVED, NW>())) void k_fused(KP<T> p)
 {
     const int tile = xcd_tile(blockIdx.x, p.nblocks);
@@ -231,7 +266,7 @@ template <typename T, int D, bool CURVED>
 __global__ __launch_bounds__(64 * RT_FUSED_WAVES, (fused_waves_per_simd<T, D, CURVED, RT_FUSED_WAVES>())) void k_fused_batch(
     const KP<T> *__restrict__ probs, const int *__restrict__ start, int nprob, int parity)
 {
-    constexpr bool REC = false;
+    constexpr bool REC = false, SONLY = false;
     constexpr int NW = RT_FUSED_WAVES;
     int i = 0;
     while (i + 1 < nprob && start[i + 1] <= (int)blockIdx.x) i++;
@@ -288,17 +323,19 @@ KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, con
     p.rec_m = rec ? rec->m : nullptr;
     p.rec_k = rec ? rec->k : nullptr;
     p.rec_mask = rec ? rec->mask : nullptr;
+    p.f64 = c->d_f64;
+    p.alpha64 = c->d_alpha;
     return p;
 }
 
-/* Resident workgroups per CU of k_fused<T, D, CURVED, REC, NW>. */
-template <typename T, int D, bool CURVED, bool REC, int NW>
+/* Resident workgroups per CU of k_fused<T, D, CURVED, REC, NW, SONLY>. */
+template <typename T, int D, bool CURVED, bool REC, int NW, bool SONLY>
 int fused_per_cu()
 {
     static int per_cu = 0;
     if (per_cu == 0) {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<T, D, CURVED, REC, NW>, 64 * NW, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused<T, D, CURVED, REC, NW, SONLY>, 64 * NW, 0) != hipSuccess ||
             n < 1)
             n = 1;
         per_cu = n;
@@ -306,39 +343,40 @@ int fused_per_cu()
     return per_cu;
 }
 
-template <typename T, int D, bool CURVED, bool REC>
+template <typename T, int D, bool CURVED, bool REC, bool SONLY>
 int launch_one(rt_ctx *c, const void *in, void *out, bool want_moments, const RecordPtrs *rec)
 {
     /* one round of resident workgroups: each walks a long chunk of rows,
      * which amortises the two halo rows it stages first */
-    const int round4 = fused_per_cu<T, D, CURVED, REC, RT_FUSED_WAVES>() * c->ncu;
+    const int round4 = fused_per_cu<T, D, CURVED, REC, RT_FUSED_WAVES, SONLY>() * c->ncu;
     KP<T> p = make_kp<T>(c, in, out, want_moments, rec, 0, round4);
     if constexpr (has_wide<T, D>()) {
         constexpr bool one_wg = fused_waves_per_simd<T, D, CURVED, RT_FUSED_WAVES>() == 1;
         if (one_wg || 2 * p.nblocks <= round4) {
-            KP<T> p8 = make_kp<T>(c, in, out, want_moments, rec, 0, fused_per_cu<T, D, CURVED, REC, 8>() * c->ncu);
-            k_fused<T, D, CURVED, REC, 8><<<p8.nblocks, 512, 0, c->stream>>>(p8);
+            KP<T> p8 =
+                make_kp<T>(c, in, out, want_moments, rec, 0, fused_per_cu<T, D, CURVED, REC, 8, SONLY>() * c->ncu);
+            k_fused<T, D, CURVED, REC, 8, SONLY><<<p8.nblocks, 512, 0, c->stream>>>(p8);
             RT_HIP(c, hipGetLastError());
             if (REC) c->rec_waves = 8;
             return RT_OK;
         }
     }
     if (REC) c->rec_waves = RT_FUSED_WAVES;
-    k_fused<T, D, CURVED, REC, RT_FUSED_WAVES><<<p.nblocks, 64 * RT_FUSED_WAVES, 0, c->stream>>>(p);
+    k_fused<T, D, CURVED, REC, RT_FUSED_WAVES, SONLY><<<p.nblocks, 64 * RT_FUSED_WAVES, 0, c->stream>>>(p);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }
 
-template <typename T, bool CURVED, bool REC>
+template <typename T, bool CURVED, bool REC, bool SONLY>
 int dispatch_d(rt_ctx *c, const void *in, void *out, bool wm, const RecordPtrs *rec)
 {
     switch (c->D) {
-    case 16: return launch_one<T, 16, CURVED, REC>(c, in, out, wm, rec);
-    case 20: return launch_one<T, 20, CURVED, REC>(c, in, out, wm, rec);
-    case 24: return launch_one<T, 24, CURVED, REC>(c, in, out, wm, rec);
-    case 32: return launch_one<T, 32, CURVED, REC>(c, in, out, wm, rec);
-    case 48: return launch_one<T, 48, CURVED, REC>(c, in, out, wm, rec);
-    case 64: return launch_one<T, 64, CURVED, REC>(c, in, out, wm, rec);
+    case 16: return launch_one<T, 16, CURVED, REC, SONLY>(c, in, out, wm, rec);
+    case 20: return launch_one<T, 20, CURVED, REC, SONLY>(c, in, out, wm, rec);
+    case 24: return launch_one<T, 24, CURVED, REC, SONLY>(c, in, out, wm, rec);
+    case 32: return launch_one<T, 32, CURVED, REC, SONLY>(c, in, out, wm, rec);
+    case 48: return launch_one<T, 48, CURVED, REC, SONLY>(c, in, out, wm, rec);
+    case 64: return launch_one<T, 64, CURVED, REC, SONLY>(c, in, out, wm, rec);
     default: return rt_fail(c, RT_ERR_UNSUPPORTED, "fused kernel: unsupported nDir");
     }
 }
@@ -365,20 +403,41 @@ int launch_fused_step(rt_ctx *c, const void *in, void *out, bool want_moments)
 {
     const bool curved = c->desc.streaming == RT_CURVED_FIXED;
     if (c->fp32)
-        return curved ? dispatch_d<float, true, false>(c, in, out, want_moments, nullptr)
-                      : dispatch_d<float, false, false>(c, in, out, want_moments, nullptr);
-    return curved ? dispatch_d<double, true, false>(c, in, out, want_moments, nullptr)
-                  : dispatch_d<double, false, false>(c, in, out, want_moments, nullptr);
+        return curved ? dispatch_d<float, true, false, false>(c, in, out, want_moments, nullptr)
+                      : dispatch_d<float, false, false, false>(c, in, out, want_moments, nullptr);
+    return curved ? dispatch_d<double, true, false, false>(c, in, out, want_moments, nullptr)
+                  : dispatch_d<double, false, false, false>(c, in, out, want_moments, nullptr);
+}
+
+/* FP64: the fused stream into the true-unit buffer J (d_J), then the exact
+ * collide (kernels_collide.hip) into out. */
+int launch_exact_step(rt_ctx *c, const void *in, void *out, bool want_moments, const RecordPtrs *rec)
+{
+    const bool curved = c->desc.streaming == RT_CURVED_FIXED;
+    if (!c->d_J) return rt_fail(c, RT_ERR_STATE, "exact FP64 step: no streaming buffer");
+    int r;
+    if (rec)
+        r = curved ? dispatch_d<double, true, true, true>(c, in, c->d_J, false, rec)
+                   : dispatch_d<double, false, true, true>(c, in, c->d_J, false, rec);
+    else
+        r = curved ? dispatch_d<double, true, false, true>(c, in, c->d_J, false, nullptr)
+                   : dispatch_d<double, false, false, true>(c, in, c->d_J, false, nullptr);
+    if (r != RT_OK) return r;
+    return launch_collide_exact(c, c->d_J, out, want_moments, rec);
+}
+
+int launch_product_step(rt_ctx *c, const void *in, void *out, bool want_moments)
+{
+    if (!c->fp32) return launch_exact_step(c, in, out, want_moments, nullptr);
+    return launch_fused_step(c, in, out, want_moments);
 }
 
 int launch_fused_step_rec(rt_ctx *c, const void *in, void *out, const RecordPtrs &rec)
 {
     const bool curved = c->desc.streaming == RT_CURVED_FIXED;
-    if (c->fp32)
-        return curved ? dispatch_d<float, true, true>(c, in, out, false, &rec)
-                      : dispatch_d<float, false, true>(c, in, out, false, &rec);
-    return curved ? dispatch_d<double, true, true>(c, in, out, false, &rec)
-                  : dispatch_d<double, false, true>(c, in, out, false, &rec);
+    if (!c->fp32) return launch_exact_step(c, in, out, false, &rec);
+    return curved ? dispatch_d<float, true, true, false>(c, in, out, false, &rec)
+                  : dispatch_d<float, false, true, false>(c, in, out, false, &rec);
 }
 
 /* The fused kernel's form of the Fourier tables (computed once per table
@@ -493,10 +552,26 @@ int launch_group_t(rt_batch *b, const rt_batch::Group &g, int parity)
     return RT_OK;
 }
 
+/* FP64 members step as rt_step does (fused stream, exact collide), one
+ * member after another on the batch's stream. */
+int launch_group_exact(rt_batch *b, const rt_batch::Group &g, int parity)
+{
+    for (int m : g.members) {
+        rt_ctx *c = b->ctx[m];
+        void *in = c->d_I[parity ? 1 - c->cur : c->cur], *out = c->d_I[parity ? c->cur : 1 - c->cur];
+        hipStream_t own = c->stream;
+        c->stream = b->stream;
+        const int r = launch_exact_step(c, in, out, false, nullptr);
+        c->stream = own;
+        if (r != RT_OK) return bfail(b, r, std::string("batch member: ") + c->err);
+    }
+    return RT_OK;
+}
+
 int launch_group(rt_batch *b, const rt_batch::Group &g, int parity)
 {
     if (g.fp32) return g.curved ? launch_group_t<float, true>(b, g, parity) : launch_group_t<float, false>(b, g, parity);
-    return g.curved ? launch_group_t<double, true>(b, g, parity) : launch_group_t<double, false>(b, g, parity);
+    return launch_group_exact(b, g, parity);
 }
 
 /* One step of every problem (one launch per group). */
@@ -551,7 +626,10 @@ int capture_graph(rt_batch *b)
 }
 
 /* nSteps steps of every problem: graph replays for the pairs, one direct
- * step for an odd remainder (after which the members' current buffers flip). */
+ * step for an odd remainder (after which the members' current buffers flip:
+ * the FP32 groups read their buffers from the refreshed tables, the FP64
+ * members' launches carry them as arguments, so the graph is captured
+ * again). */
 int batch_run(rt_batch *b, int nSteps)
 {
     for (int k = 0; k < nSteps / 2; k++) BHIP(b, hipGraphLaunch(b->exec, b->stream));
@@ -559,8 +637,13 @@ int batch_run(rt_batch *b, int nSteps)
         if (int r = launch_all(b, 0)) return r;
         for (rt_ctx *c : b->ctx) c->cur = 1 - c->cur;
         BHIP(b, hipStreamSynchronize(b->stream));
-        for (auto &g : b->groups)
+        bool exact = false;
+        for (auto &g : b->groups) {
             if (int r = g.fp32 ? fill_group_table<float>(b, g) : fill_group_table<double>(b, g)) return r;
+            exact = exact || !g.fp32;
+        }
+        if (exact)
+            if (int r = capture_graph(b)) return r;
     }
     return RT_OK;
 }

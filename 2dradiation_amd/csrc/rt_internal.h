@@ -91,6 +91,9 @@ struct rt_ctx {
     bool has_I = false;
     void *d_I[2] = {nullptr, nullptr};
     int cur = 0;
+    /* FP64 fixed streaming: streamed intensities in true units [D][N], the
+     * hand-over from the fused stream to the exact collide */
+    double *d_J = nullptr;
     /* curved fused stepping keeps the state as alpha^-3 I in d_I[cur] */
     bool scaled = false;
     double *d_mom = nullptr;      /* [6][N] */
@@ -130,6 +133,9 @@ struct rt_ctx {
      * before steps 0, every, 2 every, ...), the recomputed segment, per-step
      * records, adjoint state buffers and the parameter gradients */
     bool ck_valid = false;
+    /* state convention (alpha^-3 I or not) of the checkpointed run: the
+     * checkpoints keep it even if a later call unscales the current state */
+    bool ck_scaled = false;
     int ck_steps = 0, ck_every = 0, ck_count = 0, ck_cap = 0, seg_cap = 0;
     void *d_ckpt = nullptr;
     void *d_seg = nullptr;
@@ -144,6 +150,11 @@ struct rt_ctx {
 };
 
 int rt_fail(rt_ctx *c, int code, const std::string &msg);
+#define TRY(x)                                                                                     \
+    do {                                                                                           \
+        int _r = (x);                                                                              \
+        if (_r != RT_OK) return _r;                                                                \
+    } while (0)
 int rt_hip_check(rt_ctx *c, hipError_t e, const char *what);
 #define RT_HIP(c, call)                                                                    \
     do {                                                                                   \
@@ -179,6 +190,13 @@ inline bool rt_is_curved(const rt_desc &d) { return d.streaming == RT_CURVED_FIX
 int launch_ref_moments(rt_ctx *c, const void *in);
 int launch_ref_collide(rt_ctx *c, const void *in, void *out);
 int launch_fused_step(rt_ctx *c, const void *in, void *out, bool want_moments);
+/* One step of the product path: FP32-mixed runs the fused step; FP64 runs
+ * the fused stream into d_J and the exact collide (launch_collide_exact). */
+int launch_product_step(rt_ctx *c, const void *in, void *out, bool want_moments);
+/* ComputeMomentsIF + Collide in the reference's exact arithmetic on the
+ * true-unit streamed intensities J; rec (adjoint recording) receives the
+ * moments of J and the cycle counts. */
+int launch_collide_exact(rt_ctx *c, const void *J, void *out, bool want_moments, const RecordPtrs *rec);
 bool fused_supported(const rt_ctx *c);
 /* curved state as alpha^-3 I, fixed cells equal in both buffers */
 extern "C" int prepare_fused(rt_ctx *c);

@@ -15,7 +15,8 @@
  *                                 + cc (v_{k-1}[1] Sx + v_{k-1}[2] Sy),
  *     Inew_d = q J_d + a q (eta + k0 E_{K-1}) + cc (Fx_{K-1} nx_d + Fy_{K-1} ny_d),
  *     q = 1 / (1 + a (kA + k0)),  cc = 3 k1 a q,  a = alpha dt,
- *   with the reference's stopping rule deciding K per cell.  Halo and excised
+ *   with K per cell the cycle count of the reference's own per-direction
+ *   cycle on the same J (or_collide_cell).  Halo and excised
  *   cells keep their values (they are constants and carry no gradient).
  * The cycle count K is held fixed in the derivative (the rule is piecewise
  * constant).  Gradients: reverse mode through the moment recursion for J,
@@ -27,6 +28,7 @@
 #include "rt_oracle.h"
 
 #define ADJ_PI 3.14159265358979323846
+#define OR_ADJ_MAXDIR 4096
 
 /* One cell-direction gather written as a list of (state index, weight) taps
  * and per-corner clamps: out = scale * sum_c bw[c] * r3[c] * max(0, pre[c]),
@@ -173,24 +175,25 @@ static void collide_cell(const or_sim *s, const double S[3][6], long ij, const d
     }
     const double mJ[6] = {e, fx, fy, pxx, pxy, pyy};
     memcpy(c->mJ, mJ, sizeof mJ);
+    /* the cycle count is the reference's: its per-direction cycle decides
+     * it (a moment that is rounding noise, exactly 0 in some cells, stops
+     * the loop through 0/0), and the recursion below runs that many cycles */
+    double mref[6], Iscr[OR_ADJ_MAXDIR];
+    memcpy(mref, mJ, sizeof mref);
+    const int Kref = or_collide_cell(s, ij, J + ij * D, Iscr, mref);
     double prev[6], te_prev[3] = {0, 0, 0}, tk_prev[3] = {0, 0, 0};
     memcpy(prev, mJ, sizeof prev);
     double v[3] = {mJ[0], mJ[1], mJ[2]}, te[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
     int K = 0;
-    double diff = 1;
-    while (fabs(diff) > 1e-4 && K < 100) {
+    while (K < Kref) {
         K++;
         /* the moments entering this cycle and their tangents */
         v[0] = prev[0]; v[1] = prev[1]; v[2] = prev[2];
         memcpy(te, te_prev, sizeof te);
         memcpy(tk, tk_prev, sizeof tk);
         const double b = a * c->q * (c->eta + c->k0 * prev[0]);
-        double nw[6], ss = 0;
-        for (int X = 0; X < 6; X++) {
-            nw[X] = c->q * mJ[X] + b * S[0][X] + c->cc * (prev[1] * S[1][X] + prev[2] * S[2][X]);
-            const double r = (prev[X] - nw[X]) / nw[X];
-            ss += r * r;
-        }
+        double nw[6];
+        for (int X = 0; X < 6; X++) nw[X] = c->q * mJ[X] + b * S[0][X] + c->cc * (prev[1] * S[1][X] + prev[2] * S[2][X]);
         /* tangents of the new (E, Fx, Fy) */
         double ne[3], nk[3];
         for (int X = 0; X < 3; X++) {
@@ -203,7 +206,6 @@ static void collide_cell(const or_sim *s, const double S[3][6], long ij, const d
         memcpy(te_prev, ne, sizeof ne);
         memcpy(tk_prev, nk, sizeof nk);
         memcpy(prev, nw, sizeof prev);
-        diff = sqrt(ss);
     }
     c->K = K;
     memcpy(c->v, v, sizeof v);

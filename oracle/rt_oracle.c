@@ -723,68 +723,81 @@ void or_moments_lf(const or_grid *g, const double *mom, const double *tetrad, co
 }
 /* Collide, Radiation.cpp:641-724.  Writes Inew at interior active cells,
  * updates the moment arrays in place, and records cycle counts. */
+/* Collide of one active cell (src/Radiation.cpp:650-724): Icell and
+ * Inew_cell hold its nDir intensities; m holds E, Fx, Fy, Pxx, Pxy, Pyy of
+ * Icell on entry and the moments of the last cycle on return.  Returns the
+ * cycle count. */
+int or_collide_cell(const or_sim *s, long ij, const double *Icell, double *Inew_cell, double m[6])
+{
+    const or_grid *g = &s->grid;
+    const int D = s->nDir;
+    double ux = s->ux[ij], uy = s->uy[ij];
+    double u1 = s->t11[ij] * ux + s->t12[ij] * uy;
+    double u2 = s->t21[ij] * ux + s->t22[ij] * uy;
+    double alpha = s->alpha[ij];
+    double uu = u1 * u1 + u2 * u2;
+    double lorentz = 1.0 / sqrt(1.0 - uu);
+    const double rs = s->rot ? or_mysin(s->rot[ij]) : 0.0, rc = s->rot ? or_mycos(s->rot[ij]) : 1.0;
+    int cycles = 0;
+    double diff = 1;
+    while (fabs(diff) > 1e-4 && (cycles < 100)) {
+        cycles++;
+        double pE = m[0], pFx = m[1], pFy = m[2], pPxx = m[3], pPxy = m[4], pPyy = m[5];
+        double uF = u1 * pFx + u2 * pFy;
+        double uxP = u1 * pPxx + u2 * pPxy;
+        double uyP = u1 * pPxy + u2 * pPyy;
+        double uuP = u1 * uxP + u2 * uyP;
+        double fE = lorentz * lorentz * (pE - 2.0 * uF + uuP);
+        double fFx = lorentz * lorentz * (pFx - lorentz * pE * u1 - uxP + u1 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
+        double fFy = lorentz * lorentz * (pFy - lorentz * pE * u2 - uyP + u2 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
+        double e = 0, fx = 0, fy = 0, pxx = 0, pxy = 0, pyy = 0;
+        for (int d = 0; d < D; d++) {
+            double n1 = s->cx[d], n2 = s->cy[d];
+            if (s->rot) { /* nIF = RotationMatrix(rotationAngle[ij]) * C(d) (:691) */
+                n1 = s->cx[d] * rc - s->cy[d] * rs;
+                n2 = s->cx[d] * rs + s->cy[d] * rc;
+            }
+            double un = u1 * n1 + u2 * n2;
+            double A = lorentz * (1.0 - un);
+            /* Tensor2 nFF = (nIF - (1 - (lorentz*un)/(lorentz+1)) * lorentz * uIF) / A */
+            double fac = (1 - (lorentz * un) / (lorentz + 1)) * lorentz;
+            double nf1 = (n1 - fac * u1) / A, nf2 = (n2 - fac * u2) / A;
+            double nF = nf1 * fFx + nf2 * fFy;
+            double M = s->kappa0[ij] * fE + 3.0 * s->kappa1[ij] * nF;
+            double v = (Icell[d] + alpha * g->dt * (s->eta[ij] + M) / (A * A)) /
+                       (1.0 + alpha * g->dt * (s->kappaA[ij] + s->kappa0[ij]) * A);
+            Inew_cell[d] = v;
+            double c = s->w[d] * v;
+            e += c;
+            fx += c * n1;
+            fy += c * n2;
+            pxx += c * n1 * n1;
+            pxy += c * n1 * n2;
+            pyy += c * n2 * n2;
+        }
+        m[0] = e; m[1] = fx; m[2] = fy; m[3] = pxx; m[4] = pxy; m[5] = pyy;
+        double dE = (pE - e) / e, dFx = (pFx - fx) / fx, dFy = (pFy - fy) / fy;
+        double dPxx = (pPxx - pxx) / pxx, dPxy = (pPxy - pxy) / pxy, dPyy = (pPyy - pyy) / pyy;
+        diff = sqrt(dE * dE + dFx * dFx + dFy * dFy + dPxx * dPxx + dPxy * dPxy + dPyy * dPyy);
+    }
+    return cycles;
+}
+
 void or_collide(const or_sim *s, const double *I, double *Inew, double *mom, int32_t *count)
 {
     const or_grid *g = &s->grid;
     const long N = (long)g->nx * g->ny;
     const int D = s->nDir;
-    double *E = mom, *Fx = mom + N, *Fy = mom + 2 * N, *Pxx = mom + 3 * N, *Pxy = mom + 4 * N, *Pyy = mom + 5 * N;
 #pragma omp parallel for collapse(2) schedule(dynamic, 50)
     for (int j = 1; j < g->ny - 1; j++)
         for (int i = 1; i < g->nx - 1; i++) {
             double xc = g->startx + i * g->dx, yc = g->starty + j * g->dy;
             if (or_inside_bh(s->bh_r2, xc, yc)) continue;
             long ij = i + (long)j * g->nx;
-            double ux = s->ux[ij], uy = s->uy[ij];
-            double u1 = s->t11[ij] * ux + s->t12[ij] * uy;
-            double u2 = s->t21[ij] * ux + s->t22[ij] * uy;
-            double alpha = s->alpha[ij];
-            double uu = u1 * u1 + u2 * u2;
-            double lorentz = 1.0 / sqrt(1.0 - uu);
-            const double rs = s->rot ? or_mysin(s->rot[ij]) : 0.0, rc = s->rot ? or_mycos(s->rot[ij]) : 1.0;
-            int cycles = 0;
-            double diff = 1;
-            while (fabs(diff) > 1e-4 && (cycles < 100)) {
-                cycles++;
-                double pE = E[ij], pFx = Fx[ij], pFy = Fy[ij], pPxx = Pxx[ij], pPxy = Pxy[ij], pPyy = Pyy[ij];
-                double uF = u1 * pFx + u2 * pFy;
-                double uxP = u1 * pPxx + u2 * pPxy;
-                double uyP = u1 * pPxy + u2 * pPyy;
-                double uuP = u1 * uxP + u2 * uyP;
-                double fE = lorentz * lorentz * (pE - 2.0 * uF + uuP);
-                double fFx = lorentz * lorentz * (pFx - lorentz * pE * u1 - uxP + u1 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
-                double fFy = lorentz * lorentz * (pFy - lorentz * pE * u2 - uyP + u2 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
-                double e = 0, fx = 0, fy = 0, pxx = 0, pxy = 0, pyy = 0;
-                for (int d = 0; d < D; d++) {
-                    long idx = ij * D + d;
-                    double n1 = s->cx[d], n2 = s->cy[d];
-                    if (s->rot) { /* nIF = RotationMatrix(rotationAngle[ij]) * C(d) (:691) */
-                        n1 = s->cx[d] * rc - s->cy[d] * rs;
-                        n2 = s->cx[d] * rs + s->cy[d] * rc;
-                    }
-                    double un = u1 * n1 + u2 * n2;
-                    double A = lorentz * (1.0 - un);
-                    /* Tensor2 nFF = (nIF - (1 - (lorentz*un)/(lorentz+1)) * lorentz * uIF) / A */
-                    double fac = (1 - (lorentz * un) / (lorentz + 1)) * lorentz;
-                    double nf1 = (n1 - fac * u1) / A, nf2 = (n2 - fac * u2) / A;
-                    double nF = nf1 * fFx + nf2 * fFy;
-                    double M = s->kappa0[ij] * fE + 3.0 * s->kappa1[ij] * nF;
-                    double v = (I[idx] + alpha * g->dt * (s->eta[ij] + M) / (A * A)) /
-                               (1.0 + alpha * g->dt * (s->kappaA[ij] + s->kappa0[ij]) * A);
-                    Inew[idx] = v;
-                    double c = s->w[d] * v;
-                    e += c;
-                    fx += c * n1;
-                    fy += c * n2;
-                    pxx += c * n1 * n1;
-                    pxy += c * n1 * n2;
-                    pyy += c * n2 * n2;
-                }
-                E[ij] = e; Fx[ij] = fx; Fy[ij] = fy; Pxx[ij] = pxx; Pxy[ij] = pxy; Pyy[ij] = pyy;
-                double dE = (pE - e) / e, dFx = (pFx - fx) / fx, dFy = (pFy - fy) / fy;
-                double dPxx = (pPxx - pxx) / pxx, dPxy = (pPxy - pxy) / pxy, dPyy = (pPyy - pyy) / pyy;
-                diff = sqrt(dE * dE + dFx * dFx + dFy * dFy + dPxx * dPxx + dPxy * dPxy + dPyy * dPyy);
-            }
+            double m[6];
+            for (int q = 0; q < 6; q++) m[q] = mom[q * N + ij];
+            const int cycles = or_collide_cell(s, ij, I + ij * D, Inew + ij * D, m);
+            for (int q = 0; q < 6; q++) mom[q * N + ij] = m[q];
             if (count) count[ij] = cycles;
         }
 }

@@ -103,6 +103,10 @@ void or_moments_lf(const or_grid *g, const double *mom /*[6][N]*/, const double 
                    const double *gamma /*[3][N]*/, double r2, double *out /*[7][N]*/);
 /* Collide in place on the moment arrays; writes Inew for interior active cells. */
 void or_collide(const or_sim *s, const double *I, double *Inew, double *mom, int32_t *count);
+/* Collide of one active cell ij (src/Radiation.cpp:650-724): Icell/Inew_cell
+ * are its nDir intensities, m its six moments (in: of Icell; out: after the
+ * last cycle).  Returns the cycle count. */
+int or_collide_cell(const or_sim *s, long ij, const double *Icell, double *Inew_cell, double m[6]);
 
 /* ---- Adaptive streaming (rt_adaptive.c) ---- */
 double or_myacos(double x);                                /* src/Utility.hh:453-471 */

@@ -138,3 +138,46 @@ def test_adjoint_rejects_bad_calls(rt, O):
         ctx.forward_checkpointed(4, 2)
     assert e.value.code == rt.RT_ERR_UNSUPPORTED
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_adjoint_after_download_keeps_checkpoint_convention(rt, O):
+    """Curved FP64: downloading the state between the checkpointed forward and
+    the adjoint turns the current state back into intensities; the adjoint
+    still works in the convention its checkpoints were taken in."""
+    kind, n, L, D, steps, every, R = CASES[1]
+    px, py = ring(32, R)
+    P, ctx, g, fields, I0 = build(rt, O, kind, n, L, D, rt.FP64)
+    states = P.adj_forward(I0, steps)
+    _, _, lamT = P.observer_loss(states[-1], px, py, np.zeros((32, D)), want_grad=True)
+    ge_ref, gk_ref = P.adjoint(states, lamT)
+    ctx.upload_I(I0)
+    ctx.forward_checkpointed(steps, every)
+    obs = ctx.sample_outgoing(px, py)
+    I = ctx.download_I()
+    assert rel_l2(I, states[-1]) <= 1e-12
+    ge, gk = ctx.adjoint(px, py, obs)
+    assert rel_l2(ge, ge_ref) <= 1e-9
+    assert rel_l2(gk, gk_ref) <= 1e-9
+    # and the current state is still the downloaded one
+    assert rel_l2(ctx.download_I(), I) == 0.0
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_autograd_backward_after_another_forward_raises(rt, O):
+    import torch
+    ag = importlib.import_module("2dradiation_amd.autograd")
+    P, ctx, g, fields, I0 = build(rt, O, 1, 40, 4.0, 16, rt.FP64)
+    px, py = ring(32, 3.0)
+    extra = {"kappa1": fields["kappa1"]}
+    eta = torch.tensor(fields["eta"], requires_grad=True)
+    k0 = torch.tensor(fields["kappa0"], requires_grad=True)
+    out1 = ag.observer_ring(eta, k0, ctx, I0, 6, 3, px, py, extra)
+    out2 = ag.observer_ring(eta, k0, ctx, I0, 8, 3, px, py, extra)
+    with pytest.raises(RuntimeError):
+        (0.5 * (out1 ** 2).sum()).backward()
+    (0.5 * (out2 ** 2).sum()).backward()
+    ge, _ = ctx.adjoint(px, py, out2.detach().numpy())
+    assert np.allclose(eta.grad.numpy(), ge, rtol=0, atol=1e-14 * np.abs(ge).max())
+    ctx.close()
