@@ -559,6 +559,7 @@ int rt_upload_I(rt_ctx *c, const double *I)
     dfree(tmp);
     if (r == RT_OK) c->has_I = true;
     c->scaled = false;
+    c->fixed_synced = false;
     return r;
 }
 
@@ -709,6 +710,8 @@ int rt_reset_iteration_stats(rt_ctx *c)
  * alpha^-3 I, and makes the fixed cells of both buffers equal again. */
 static int ensure_unscaled(rt_ctx *c)
 {
+    /* the per-method kernels below write the buffers in their own way */
+    c->fixed_synced = false;
     if (!c->scaled) return RT_OK;
     TRY(launch_scale_state(c, c->d_I[c->cur], true));
     TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
@@ -723,8 +726,12 @@ int prepare_fused(rt_ctx *c)
     if (c->desc.streaming == RT_CURVED_FIXED && !c->scaled) {
         TRY(launch_scale_state(c, c->d_I[c->cur], false));
         c->scaled = true;
+        c->fixed_synced = false;
     }
-    TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+    if (!c->fixed_synced) {
+        TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
+        c->fixed_synced = true;
+    }
     return RT_OK;
 }
 
@@ -764,6 +771,7 @@ int rt_step(rt_ctx *c, int32_t nSteps, int32_t want_moments)
         /* RunSimulation (src/Radiation.cpp:784-819): LoadInitialData for the
          * source cells, UpdateRotationMatrizes, adaptive stream (swapping
          * I and the angle buffers), ComputeMomentsIF, Collide */
+        c->fixed_synced = false;
         for (int n = 0; n < nSteps; n++) {
             TRY(launch_restore_sources(c, c->d_I[c->cur], false));
             TRY(launch_restore_source_rot(c));

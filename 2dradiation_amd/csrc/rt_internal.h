@@ -96,6 +96,9 @@ struct rt_ctx {
     double *d_J = nullptr;
     /* curved fused stepping keeps the state as alpha^-3 I in d_I[cur] */
     bool scaled = false;
+    /* the fixed (halo and excised) cells of both state buffers are equal,
+     * as the fused step needs; anything that writes them clears this */
+    bool fixed_synced = false;
     double *d_mom = nullptr;      /* [6][N] */
     /* lab frame: tetrad [9][N] and gamma_ll [3][N]; LF moments [7][N] */
     double *d_frame = nullptr;

@@ -31,8 +31,11 @@ HBM_PEAK = 8.0e12
 
 def algorithmic_bytes_per_update(nDir, b_i, b_f, curved=True, nCoeff=5):
     """2 b_I (read + write I) + (6 fields * b_f + geometry) / nDir, geometry =
-    alpha + 5 series * nCoeff coefficients in FP64 for curved spacetimes."""
-    b_geo = 8 * (1 + 5 * nCoeff) if curved else 0
+    alpha + 5 series * nCoeff coefficients in the kernel's precision for
+    curved spacetimes: the FP64 step reads the FP64 tables, the FP32-mixed
+    kernel an FP32 copy of them in relative form (S - 1, departure offsets in
+    cells), so b_geo is 8 (1 + 25) = 208 B or 4 (1 + 25) = 104 B per cell."""
+    b_geo = b_f * (1 + 5 * nCoeff) if curved else 0
     return 2 * b_i + (6 * b_f + b_geo) / nDir
 
 
@@ -51,6 +54,18 @@ def profiled_traffic_bytes():
 def rel_l2(a, b):
     n = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def cpu_model():
+    """Model name of the host CPU (lscpu's "Model name", read from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(rt, name, budget_s=12.0, max_steps=40):
@@ -83,7 +98,7 @@ def cpu_baseline(rt, name, budget_s=12.0, max_steps=40):
     elapsed = t3 - t2
     act = int(np.count_nonzero(_active(gs, r2)))
     return {"value": act * spec["nDir"] * steps / elapsed, "unit": "cell*dir updates/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"{name}, {steps} timed steps after 1 warm-up step, oracle -O3 -ffast-math OpenMP "
                       f"({threads} threads), {elapsed:.1f} s"}
 
@@ -271,7 +286,7 @@ def adaptive_cpu_baseline(rt, name, budget_s=8.0, max_steps=10):
     elapsed = time.perf_counter() - t1
     act = int(np.count_nonzero(_active(gs, r2)))
     return {"value": act * spec["nDir"] * n / elapsed, "unit": "cell*dir updates/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"{name}, {n} timed steps after 1 warm-up step, oracle -O3 -ffast-math OpenMP "
                       f"({threads} threads), {elapsed:.1f} s"}
 
@@ -328,8 +343,9 @@ def main():
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-streaming leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other single-problem configs")
     ap.add_argument("--only-adaptive", action="store_true", help="run only the adaptive-streaming leg")
-    ap.add_argument("--ref-steps", type=int, default=200,
-                    help="steps of the FP64 reference-order comparison leg (0 skips it)")
+    ap.add_argument("--ref-steps", type=int, default=2000,
+                    help="steps from I = 0 of the comparison with the FP64 reference-order kernels "
+                         "(default: the workload's full 2000; 0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -361,9 +377,10 @@ def main():
     assert c32.uses_fused(), "fused path not selected"
     zero = np.zeros(gs.N * spec["nDir"])
 
+    # W untimed steps from I = 0, then K timed steps continuing from there
+    # (no upload or state conversion between them)
     c32.step(W, want_moments=False)
     c32.synchronize()
-    c32.upload_I(zero)
     c32.reset_iteration_stats()
     c32.synchronize()
     print(f"[bench] timing {K} FP32-mixed steps", file=sys.stderr, flush=True)
@@ -377,12 +394,14 @@ def main():
         "metric": "cell*dir updates/s FP32-mixed vs FP64 at rel-L2<=1e-5; adjoint/forward ratio",
         "value": rate32, "unit": "cell*dir updates/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": ms32 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "dtype_detail": "f32 intensities and gather arithmetic; f64 tables, moment sums and lambda iteration",
+        "dtype": "f32", "dtype_detail": "f32 intensities, gather arithmetic and Fourier series (from an f32 copy "
+                                        "of the f64 geodesic tables in relative form); f64 moment sums "
+                                        "(uncompensated) and lambda iteration",
         "data": "synthetic (seeded fields)",
         "config": {"workload": WORKLOAD, "grid": "1026x1026 stored (1024x1024 interior) on [-8,8]^2",
                    "nDir": spec["nDir"], "spacetime": "Schwarzschild M=1, excision r<=2.1",
                    "cfl": spec["cfl"], "steps_per_run": K, "active_cells": act, "parallelism": "single GPU"},
-        "lambda_iterations": {"max": it_max, "mean": it_sum / max(1, act * K)},
+        "lambda_iterations": {"max": it_max, "mean": it_sum / max(1, act * K), "steps": f"{W + 1}..{W + K}"},
         "wall_s_fp32": wall32,
     }
 
@@ -401,11 +420,10 @@ def main():
         print(f"[bench] FP64 leg: {K} steps", file=sys.stderr, flush=True)
         c64, _ = rt.workloads.make_context(WORKLOAD, rt.FP64)
         c64.step(W, want_moments=False)
-        c64.upload_I(zero)
         c64.synchronize()
         ms64 = c64.time_steps(K)
         rate64 = upd_per_step * K / (ms64 * 1e-3)
-        # quality: both runs started from I=0 and took K steps
+        # quality: both runs started from I = 0 and took W + K steps
         I32, I64 = c32.download_I(), c64.download_I()
         # moments of the final state (one more step with moments requested on copies is not
         # needed: compare the state's moments computed on the host from I)
@@ -422,20 +440,24 @@ def main():
         result["fp64"] = {"value": rate64, "ms_per_step": ms64 / K,
                           "hbm_frac": B64 * upd_per_step / (ms64 / K * 1e-3) / HBM_PEAK}
         result["fp32_over_fp64"] = rate32 / rate64
-        result["rel_l2_fp32_vs_fp64"] = {"I": rel_l2(I32, I64), "E": rel_l2(E32, E64),
+        result["rel_l2_fp32_vs_fp64"] = {"steps": W + K, "I": rel_l2(I32, I64), "E": rel_l2(E32, E64),
                                          "Fx": rel_l2(Fx32, Fx64), "Fy": rel_l2(Fy32, Fy64)}
         c64.close()
         if args.ref_steps > 0:
-            # FP64 reference-order kernels (the oracle's operation order) as the
-            # FP64 truth; compared after ref_steps steps from I = 0
-            R = min(args.ref_steps, K)
+            # FP64 reference-order kernels (the oracle's operation order, one
+            # method at a time) as the FP64 truth; the product paths compared
+            # with it after ref_steps steps from I = 0
+            R = args.ref_steps
             print(f"[bench] FP64 reference-order leg: {R} steps", file=sys.stderr, flush=True)
             cr, _ = rt.workloads.make_context(WORKLOAD, rt.FP64)
             t0 = time.perf_counter()
-            for _ in range(R):
+            for k in range(R):
                 cr.stream()
                 cr.compute_moments()
                 cr.collide()
+                if (k + 1) % 500 == 0:
+                    cr.synchronize()
+                    print(f"[bench]   reference order: {k + 1} steps", file=sys.stderr, flush=True)
             cr.synchronize()
             tref = time.perf_counter() - t0
             Iref = cr.download_I()
@@ -452,8 +474,8 @@ def main():
             Eb, Fxb, Fyb = mom(J64)
             result["rel_l2_vs_reference_order_fp64"] = {
                 "steps": R, "ms_per_step_reference_order": 1e3 * tref / R,
-                "fp32_fused": {"I": rel_l2(J32, Iref), "E": rel_l2(Ea, Er), "Fx": rel_l2(Fxa, Fxr), "Fy": rel_l2(Fya, Fyr)},
-                "fp64_fused": {"I": rel_l2(J64, Iref), "E": rel_l2(Eb, Er), "Fx": rel_l2(Fxb, Fxr), "Fy": rel_l2(Fyb, Fyr)}}
+                "fp32_mixed": {"I": rel_l2(J32, Iref), "E": rel_l2(Ea, Er), "Fx": rel_l2(Fxa, Fxr), "Fy": rel_l2(Fya, Fyr)},
+                "fp64_step": {"I": rel_l2(J64, Iref), "E": rel_l2(Eb, Er), "Fx": rel_l2(Fxb, Fxr), "Fy": rel_l2(Fyb, Fyr)}}
     c32.close()
 
     if not args.no_adjoint:
