@@ -85,6 +85,62 @@ struct Tensor3x3 {
     }
 };
 
+struct rank3Indices { int i, j, k; };
+
+// Spatial 2x2 and 2x2x2 tensors with the reference's 1-based indices
+// ({1,1}..{2,2}), and the 3x3x3 spacetime tensor with 0-based ones, as the
+// metric getters return them (src/DataTypes.hh:296-360, 450-600).
+struct Tensor2x2 {
+    double data[4];
+    Tensor2x2(double v = 0) { data[0] = data[1] = data[2] = data[3] = v; }
+    Tensor2x2(double xx, double xy, double yx, double yy) : data{xx, xy, yx, yy} {}
+    double &operator[](const rank2Indices &k) { return data[(k.i - 1) * 2 + (k.j - 1)]; }
+    const double &operator[](const rank2Indices &k) const { return data[(k.i - 1) * 2 + (k.j - 1)]; }
+    Tensor2x2 operator*(const Tensor2x2 &o) const
+    {
+        return Tensor2x2(data[0] * o.data[0] + data[1] * o.data[2], data[0] * o.data[1] + data[1] * o.data[3],
+                         data[2] * o.data[0] + data[3] * o.data[2], data[2] * o.data[1] + data[3] * o.data[3]);
+    }
+    Tensor2 operator*(const Tensor2 &v) const
+    {
+        return Tensor2(data[0] * v[1] + data[1] * v[2], data[2] * v[1] + data[3] * v[2]);
+    }
+    Tensor2x2 Invert() const
+    {
+        const double id = 1.0 / (data[0] * data[3] - data[1] * data[2]);
+        return Tensor2x2(data[3] * id, -data[1] * id, -data[2] * id, data[0] * id);
+    }
+    friend std::ostream &operator<<(std::ostream &os, const Tensor2x2 &v)
+    {
+        return os << "(" << v.data[0] << "," << v.data[1] << "|" << v.data[2] << "," << v.data[3] << ")";
+    }
+};
+struct Tensor2x2x2 {
+    double data[8];
+    Tensor2x2x2(double v = 0) { for (double &x : data) x = v; }
+    Tensor2x2x2(double xxx, double xxy, double xyx, double xyy, double yxx, double yxy, double yyx, double yyy)
+        : data{xxx, xxy, xyx, xyy, yxx, yxy, yyx, yyy} {}
+    double &operator[](const rank3Indices &k) { return data[(k.i - 1) * 4 + (k.j - 1) * 2 + (k.k - 1)]; }
+    const double &operator[](const rank3Indices &k) const { return data[(k.i - 1) * 4 + (k.j - 1) * 2 + (k.k - 1)]; }
+    friend std::ostream &operator<<(std::ostream &os, const Tensor2x2x2 &v)
+    {
+        os << "(" << v.data[0] << "," << v.data[1] << "|" << v.data[2] << "," << v.data[3] << ")";
+        return os << "(" << v.data[4] << "," << v.data[5] << "|" << v.data[6] << "," << v.data[7] << ")";
+    }
+};
+struct Tensor3x3x3 {
+    double data[27];
+    Tensor3x3x3(double v = 0) { for (double &x : data) x = v; }
+    // components in index order [a][b][c], a slowest
+    template <class... A>
+    Tensor3x3x3(double first, A... rest) : data{first, (double)rest...}
+    {
+        static_assert(sizeof...(rest) == 26, "Tensor3x3x3 takes 27 components");
+    }
+    double &operator[](const rank3Indices &k) { return data[k.i * 9 + k.j * 3 + k.k]; }
+    const double &operator[](const rank3Indices &k) const { return data[k.i * 9 + k.j * 3 + k.k]; }
+};
+
 // Lorentz boost of velocity u (|u| < 1) as a 3x3 (t, x, y) matrix, the
 // identity for |u|^2 < 1e-10 (reference: BoostMatrix, src/SpecialMath.cpp:109-118).
 inline Tensor3x3 BoostMatrix(const Tensor2 &u)

@@ -51,3 +51,19 @@ void Grid::WriteFrametoCsv(float time, const DoubleBuffer &r, const DoubleBuffer
             out << r[ij] << "," << g[ij] << "," << b[ij] << "," << a[ij] << "\n";
         }
 }
+
+// Grid::Print (reference src/Grid.cpp:140-153).
+void Grid::Print() const
+{
+    std::cout << "Grid: nx=" << nx << "\n";
+    std::cout << "Grid: ny=" << ny << "\n";
+    std::cout << "Grid: nxy=" << nxy << "\n";
+    std::cout << "Grid: dx=" << dx << "\n";
+    std::cout << "Grid: dy=" << dy << "\n";
+    std::cout << "Grid: dt=" << dt << "\n";
+    std::cout << "Grid: startx=" << startx << "\n";
+    std::cout << "Grid: starty=" << starty << "\n";
+    std::cout << "Grid: endx=" << endx << "\n";
+    std::cout << "Grid: endy=" << endy << "\n";
+    std::cout << "Grid: cfl=" << m_cfl << "\n";
+}

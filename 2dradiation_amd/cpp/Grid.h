@@ -39,5 +39,8 @@ class Grid {
     {
         return xy[1] > endx || xy[1] < startx || xy[2] > endy || xy[2] < starty;
     }
+    bool OutsideDomain(double i, double j) const { return i < 0 || j < 0 || i > nx - 1 || j > ny - 1; }
+    // Grid::Print (src/Grid.cpp:140-153): the grid's sizes, one per line.
+    void Print() const;
 };
 #endif

@@ -244,3 +244,187 @@ Tensor3 Metric::uEulObs(const Coord &xy) const
     const Tensor2 b = GetBeta_u(xy);
     return Tensor3(1.0 / a, -b[1] / a, -b[2] / a);
 }
+
+// ---- remaining getters (src/Metric.cpp:406-800) ----
+namespace {
+using Arr = const DoubleBuffer *;
+// symmetric 3x3 from the arrays 00, 01, 02, 11, 12, 22
+Tensor3x3 sym3(const double v[6]) { return Tensor3x3(v[0], v[1], v[2], v[1], v[3], v[4], v[2], v[4], v[5]); }
+} // namespace
+
+Tensor3x3 Metric::GetMetric_uu(size_t ij) const
+{
+    const double v[6] = {g00_uu[ij], g01_uu[ij], g02_uu[ij], g11_uu[ij], g12_uu[ij], g22_uu[ij]};
+    return sym3(v);
+}
+Tensor3x3 Metric::GetMetric_uu(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor3x3(-1, 0, 0, 0, 1, 0, 0, 0, 1);
+    const Coord ij = grid.ij(xy);
+    const Arr a[6] = {&g00_uu, &g01_uu, &g02_uu, &g11_uu, &g12_uu, &g22_uu};
+    double v[6];
+    for (int k = 0; k < 6; k++) v[k] = InterpolateArrayTo_ij(*a[k], ij);
+    return sym3(v);
+}
+
+/* d_a g_bc with a = t (zero), x, y and the symmetric pair bc from the six
+ * arrays of each spatial derivative */
+static Tensor3x3x3 deriv_sym3(const double (&v)[2][6])
+{
+    Tensor3x3x3 t(0.0);
+    for (int a = 1; a < 3; a++) {
+        const Tensor3x3 m = sym3(v[a - 1]);
+        for (int b = 0; b < 3; b++)
+            for (int c = 0; c < 3; c++) t[{a, b, c}] = m[{b, c}];
+    }
+    return t;
+}
+Tensor3x3x3 Metric::GetDerivMetric_lll(size_t ij) const
+{
+    const double v[2][6] = {{d1_g00_lll[ij], d1_g01_lll[ij], d1_g02_lll[ij], d1_g11_lll[ij], d1_g12_lll[ij], d1_g22_lll[ij]},
+                            {d2_g00_lll[ij], d2_g01_lll[ij], d2_g02_lll[ij], d2_g11_lll[ij], d2_g12_lll[ij], d2_g22_lll[ij]}};
+    return deriv_sym3(v);
+}
+Tensor3x3x3 Metric::GetDerivMetric_lll(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor3x3x3(0.0);
+    const Coord ij = grid.ij(xy);
+    const Arr a[2][6] = {{&d1_g00_lll, &d1_g01_lll, &d1_g02_lll, &d1_g11_lll, &d1_g12_lll, &d1_g22_lll},
+                         {&d2_g00_lll, &d2_g01_lll, &d2_g02_lll, &d2_g11_lll, &d2_g12_lll, &d2_g22_lll}};
+    double v[2][6];
+    for (int d = 0; d < 2; d++)
+        for (int k = 0; k < 6; k++) v[d][k] = InterpolateArrayTo_ij(*a[d][k], ij);
+    return deriv_sym3(v);
+}
+Tensor3x3x3 Metric::GetDerivMetric_luu(size_t ij) const
+{
+    const double v[2][6] = {{d1_g00_luu[ij], d1_g01_luu[ij], d1_g02_luu[ij], d1_g11_luu[ij], d1_g12_luu[ij], d1_g22_luu[ij]},
+                            {d2_g00_luu[ij], d2_g01_luu[ij], d2_g02_luu[ij], d2_g11_luu[ij], d2_g12_luu[ij], d2_g22_luu[ij]}};
+    return deriv_sym3(v);
+}
+Tensor3x3x3 Metric::GetDerivMetric_luu(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor3x3x3(0.0);
+    const Coord ij = grid.ij(xy);
+    const Arr a[2][6] = {{&d1_g00_luu, &d1_g01_luu, &d1_g02_luu, &d1_g11_luu, &d1_g12_luu, &d1_g22_luu},
+                         {&d2_g00_luu, &d2_g01_luu, &d2_g02_luu, &d2_g11_luu, &d2_g12_luu, &d2_g22_luu}};
+    double v[2][6];
+    for (int d = 0; d < 2; d++)
+        for (int k = 0; k < 6; k++) v[d][k] = InterpolateArrayTo_ij(*a[d][k], ij);
+    return deriv_sym3(v);
+}
+
+Tensor2 Metric::GetBeta_l(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor2(0.0);
+    const Coord ij = grid.ij(xy);
+    return Tensor2(InterpolateArrayTo_ij(beta1_l, ij), InterpolateArrayTo_ij(beta2_l, ij));
+}
+
+/* 2x2 from four arrays (xx, xy, yx, yy) at cell ij or at a point; outside
+ * the domain the given flat value */
+static Tensor2x2 mat2_at(const Metric &m, const Arr (&a)[4], const Coord &xy, const Tensor2x2 &outside)
+{
+    if (m.grid.OutsideDomain(xy)) return outside;
+    const Coord ij = m.grid.ij(xy);
+    return Tensor2x2(m.InterpolateArrayTo_ij(*a[0], ij), m.InterpolateArrayTo_ij(*a[1], ij),
+                     m.InterpolateArrayTo_ij(*a[2], ij), m.InterpolateArrayTo_ij(*a[3], ij));
+}
+static Tensor2x2 mat2_ij(const Arr (&a)[4], size_t ij) { return Tensor2x2((*a[0])[ij], (*a[1])[ij], (*a[2])[ij], (*a[3])[ij]); }
+
+Tensor2x2 Metric::GetGamma_ll(size_t ij) const
+{
+    const Arr a[4] = {&gamma11_ll, &gamma12_ll, &gamma12_ll, &gamma22_ll};
+    return mat2_ij(a, ij);
+}
+Tensor2x2 Metric::GetGamma_ll(const Coord &xy) const
+{
+    const Arr a[4] = {&gamma11_ll, &gamma12_ll, &gamma12_ll, &gamma22_ll};
+    return mat2_at(*this, a, xy, Tensor2x2(1, 0, 0, 1));
+}
+Tensor2x2 Metric::GetGamma_uu(size_t ij) const
+{
+    const Arr a[4] = {&gamma11_uu, &gamma12_uu, &gamma12_uu, &gamma22_uu};
+    return mat2_ij(a, ij);
+}
+Tensor2x2 Metric::GetGamma_uu(const Coord &xy) const
+{
+    const Arr a[4] = {&gamma11_uu, &gamma12_uu, &gamma12_uu, &gamma22_uu};
+    return mat2_at(*this, a, xy, Tensor2x2(1, 0, 0, 1));
+}
+Tensor2 Metric::GetDerivAlpha_l(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor2(0.0);
+    const Coord ij = grid.ij(xy);
+    return Tensor2(InterpolateArrayTo_ij(d1_alpha_l, ij), InterpolateArrayTo_ij(d2_alpha_l, ij));
+}
+Tensor2x2 Metric::GetDerivBeta_lu(size_t ij) const
+{
+    const Arr a[4] = {&d1_beta1_lu, &d1_beta2_lu, &d2_beta1_lu, &d2_beta2_lu};
+    return mat2_ij(a, ij);
+}
+Tensor2x2 Metric::GetDerivBeta_lu(const Coord &xy) const
+{
+    const Arr a[4] = {&d1_beta1_lu, &d1_beta2_lu, &d2_beta1_lu, &d2_beta2_lu};
+    return mat2_at(*this, a, xy, Tensor2x2(0.0));
+}
+Tensor2x2 Metric::GetDerivBeta_ll(size_t ij) const
+{
+    const Arr a[4] = {&d1_beta1_ll, &d1_beta2_ll, &d2_beta1_ll, &d2_beta2_ll};
+    return mat2_ij(a, ij);
+}
+Tensor2x2 Metric::GetDerivBeta_ll(const Coord &xy) const
+{
+    const Arr a[4] = {&d1_beta1_ll, &d1_beta2_ll, &d2_beta1_ll, &d2_beta2_ll};
+    return mat2_at(*this, a, xy, Tensor2x2(0.0));
+}
+Tensor2x2 Metric::GetExtrCurv_ll(size_t ij) const
+{
+    const Arr a[4] = {&K11_ll, &K12_ll, &K12_ll, &K22_ll};
+    return mat2_ij(a, ij);
+}
+Tensor2x2 Metric::GetExtrCurv_ll(const Coord &xy) const
+{
+    const Arr a[4] = {&K11_ll, &K12_ll, &K12_ll, &K22_ll};
+    return mat2_at(*this, a, xy, Tensor2x2(0.0));
+}
+
+/* d_a gamma_bc for a, b, c in {1, 2} */
+static Tensor2x2x2 deriv_sym2(const double (&v)[2][3])
+{
+    return Tensor2x2x2(v[0][0], v[0][1], v[0][1], v[0][2], v[1][0], v[1][1], v[1][1], v[1][2]);
+}
+Tensor2x2x2 Metric::GetDerivGamma_lll(size_t ij) const
+{
+    const double v[2][3] = {{d1_gamma11_lll[ij], d1_gamma12_lll[ij], d1_gamma22_lll[ij]},
+                            {d2_gamma11_lll[ij], d2_gamma12_lll[ij], d2_gamma22_lll[ij]}};
+    return deriv_sym2(v);
+}
+Tensor2x2x2 Metric::GetDerivGamma_lll(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor2x2x2(0.0);
+    const Coord ij = grid.ij(xy);
+    const Arr a[2][3] = {{&d1_gamma11_lll, &d1_gamma12_lll, &d1_gamma22_lll},
+                         {&d2_gamma11_lll, &d2_gamma12_lll, &d2_gamma22_lll}};
+    double v[2][3];
+    for (int d = 0; d < 2; d++)
+        for (int k = 0; k < 3; k++) v[d][k] = InterpolateArrayTo_ij(*a[d][k], ij);
+    return deriv_sym2(v);
+}
+Tensor2x2x2 Metric::GetDerivGamma_luu(size_t ij) const
+{
+    const double v[2][3] = {{d1_gamma11_luu[ij], d1_gamma12_luu[ij], d1_gamma22_luu[ij]},
+                            {d2_gamma11_luu[ij], d2_gamma12_luu[ij], d2_gamma22_luu[ij]}};
+    return deriv_sym2(v);
+}
+Tensor2x2x2 Metric::GetDerivGamma_luu(const Coord &xy) const
+{
+    if (grid.OutsideDomain(xy)) return Tensor2x2x2(0.0);
+    const Coord ij = grid.ij(xy);
+    const Arr a[2][3] = {{&d1_gamma11_luu, &d1_gamma12_luu, &d1_gamma22_luu},
+                         {&d2_gamma11_luu, &d2_gamma12_luu, &d2_gamma22_luu}};
+    double v[2][3];
+    for (int d = 0; d < 2; d++)
+        for (int k = 0; k < 3; k++) v[d][k] = InterpolateArrayTo_ij(*a[d][k], ij);
+    return deriv_sym2(v);
+}

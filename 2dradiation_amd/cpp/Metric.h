@@ -60,6 +60,44 @@ class Metric {
     // four-velocity of the Eulerian observer (src/Metric.cpp:368-382)
     Tensor3 uEulObs(size_t ij) const;
     Tensor3 uEulObs(const Coord &xy) const;
+
+    // The remaining getters of the reference (src/Metric.h:86-137,
+    // src/Metric.cpp:406-800): the value of cell ij, or at a point the
+    // bilinear value inside the domain and flat space outside it (zero
+    // derivatives and extrinsic curvature).  Time derivatives are zero for
+    // the static spacetimes here.
+    Tensor3x3 GetMetric_uu(size_t ij) const;
+    Tensor3x3 GetMetric_uu(const Coord &xy) const;
+    Tensor3x3 GetMinkowskiMetric_ll(size_t) const { return Tensor3x3(-1, 0, 0, 0, 1, 0, 0, 0, 1); }
+    Tensor3x3 GetMinkowskiMetric_ll(const Coord &) const { return Tensor3x3(-1, 0, 0, 0, 1, 0, 0, 0, 1); }
+    Tensor3x3 GetMinkowskiMetric_uu(size_t) const { return Tensor3x3(-1, 0, 0, 0, 1, 0, 0, 0, 1); }
+    Tensor3x3 GetMinkowskiMetric_uu(const Coord &) const { return Tensor3x3(-1, 0, 0, 0, 1, 0, 0, 0, 1); }
+    Tensor3x3x3 GetDerivMetric_lll(size_t ij) const;
+    Tensor3x3x3 GetDerivMetric_lll(const Coord &xy) const;
+    Tensor3x3x3 GetDerivMetric_luu(size_t ij) const;
+    Tensor3x3x3 GetDerivMetric_luu(const Coord &xy) const;
+    Tensor2 GetBeta_l(size_t ij) const { return Tensor2(beta1_l[ij], beta2_l[ij]); }
+    Tensor2 GetBeta_l(const Coord &xy) const;
+    Tensor2x2 GetGamma_ll(size_t ij) const;
+    Tensor2x2 GetGamma_ll(const Coord &xy) const;
+    Tensor2x2 GetGamma_uu(size_t ij) const;
+    Tensor2x2 GetGamma_uu(const Coord &xy) const;
+    Tensor2x2 GetMinkowskiGamma_ll(size_t) const { return Tensor2x2(1, 0, 0, 1); }
+    Tensor2x2 GetMinkowskiGamma_ll(const Coord &) const { return Tensor2x2(1, 0, 0, 1); }
+    Tensor2x2 GetMinkowskiGamma_uu(size_t) const { return Tensor2x2(1, 0, 0, 1); }
+    Tensor2x2 GetMinkowskiGamma_uu(const Coord &) const { return Tensor2x2(1, 0, 0, 1); }
+    Tensor2 GetDerivAlpha_l(size_t ij) const { return Tensor2(d1_alpha_l[ij], d2_alpha_l[ij]); }
+    Tensor2 GetDerivAlpha_l(const Coord &xy) const;
+    Tensor2x2 GetDerivBeta_lu(size_t ij) const;
+    Tensor2x2 GetDerivBeta_lu(const Coord &xy) const;
+    Tensor2x2 GetDerivBeta_ll(size_t ij) const;
+    Tensor2x2 GetDerivBeta_ll(const Coord &xy) const;
+    Tensor2x2x2 GetDerivGamma_lll(size_t ij) const;
+    Tensor2x2x2 GetDerivGamma_lll(const Coord &xy) const;
+    Tensor2x2x2 GetDerivGamma_luu(size_t ij) const;
+    Tensor2x2x2 GetDerivGamma_luu(const Coord &xy) const;
+    Tensor2x2 GetExtrCurv_ll(size_t ij) const;
+    Tensor2x2 GetExtrCurv_ll(const Coord &xy) const;
     // Per-cell block [nxy * RT_NMETRIC] consumed by rt_build_fourier_tables.
     void PackGeodesicBlock(double *out) const;
 

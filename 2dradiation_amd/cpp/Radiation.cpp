@@ -134,6 +134,61 @@ void Radiation::DownloadMoments()
     Check(rt_get_moments(ctx, E.data(), Fx.data(), Fy.data(), Pxx.data(), Pxy.data(), Pyy.data()));
 }
 
+// Reference: src/Radiation.cpp:303-320 (Fourier::GetValue,
+// src/FourierHarmonics.cpp:56-62, of the cell's series at the angle).
+static double SeriesValue(const DoubleBuffer &coefficients, size_t first, size_t nF, double angle)
+{
+    double v = 0;
+    for (size_t k = 0; k < nF; k++) v += coefficients[first + k] * rtm::fourier_basis((int)k, angle);
+    return v;
+}
+Coord Radiation::GetTempCoordinate(size_t ij, double angle) const
+{
+    const size_t nF = streamingStencil.nCoefficients, f0 = HarmonicIndex(0, ij);
+    return Coord(SeriesValue(coefficientsX, f0, nF, angle), SeriesValue(coefficientsY, f0, nF, angle));
+}
+Tensor2 Radiation::GetTemp2VelocityIF(size_t ij, double angle) const
+{
+    const size_t nF = streamingStencil.nCoefficients, f0 = HarmonicIndex(0, ij);
+    return Tensor2(SeriesValue(coefficientsCx, f0, nF, angle), SeriesValue(coefficientsCy, f0, nF, angle));
+}
+double Radiation::GetFrequencyShift(size_t ij, double angle) const
+{
+    return SeriesValue(coefficientsS, HarmonicIndex(0, ij), streamingStencil.nCoefficients, angle);
+}
+
+// Reference: src/Radiation.cpp:322-343: the velocity turned into the cell's
+// frame, the fine angle, two 4-point cubic sums, their lerp, clamped at 0.
+double Radiation::IntensityAt(size_t ij, Tensor2 vTempIF) const
+{
+    vTempIF = RotationMatrix(rotationAngle[ij]).Inverse() * vTempIF;
+    const InterpolationGrid &g = stencil.interpolationGrid;
+    const double d = g.d(vTempIF.Phi());
+    const size_t d0 = (size_t)((int)std::floor(d) % (int)g.nGrid), d1 = (d0 + 1) % g.nGrid;
+    double v0 = 0, v1 = 0;
+    for (size_t k = 0; k < 4; k++) {
+        v0 += g.neighbourWeightsCubic[d0][k] * I[Index(ij, g.neighbourIndexesCubic[d0][k])];
+        v1 += g.neighbourWeightsCubic[d1][k] * I[Index(ij, g.neighbourIndexesCubic[d1][k])];
+    }
+    const double t = d - d0;
+    return std::max(0.0, v0 * (1.0 - t) + v1 * t);
+}
+
+// Reference: src/Radiation.cpp:345-359.
+Tensor2 Radiation::AverageF(size_t i, size_t j) const
+{
+    Tensor2 f(0.0);
+    for (int b = -1; b <= 1; b++)
+        for (int a = -1; a <= 1; a++) {
+            const size_t k = grid.Index(i + a, j + b);
+            f[1] += Fx[k];
+            f[2] += Fy[k];
+        }
+    f[1] /= 9.0;
+    f[2] /= 9.0;
+    return f;
+}
+
 // Reference: src/Radiation.cpp:103-123 (InitialDataLFtoIF).  The lab-frame
 // tensor is transformed with the inverse tetrad; as in the reference only
 // its energy density is taken from the result, the flux from the lab-frame

@@ -51,8 +51,6 @@ class Radiation {
     void UploadRotation();
     void DownloadRotation();
     std::vector<double> sourceAngle; // angle LoadInitialData wrote per cell, NaN: none
-    Tensor3 InitialDataLFtoIF(size_t ij) const;
-    double NormalizationOverwrite() const;
 
   public:
     Grid &grid;
@@ -93,6 +91,21 @@ class Radiation {
     size_t Index(size_t ij, size_t d) const { return d + ij * stencil.nDir; }
     size_t Index(size_t i, size_t j, size_t d) const { return d + grid.Index(i, j) * stencil.nDir; }
     size_t HarmonicIndex(size_t f, size_t ij) const { return f + ij * streamingStencil.nCoefficients; }
+
+    // Host-side helpers of the reference (src/Radiation.cpp:303-359), on the
+    // host buffers: the departure point, departure velocity and frequency
+    // shift series of cell ij at an angle (from coefficients{X,Y,Cx,Cy,S},
+    // which RunSimulation and UpdateFourierCoefficients fill), the angular
+    // interpolation of I at a velocity in the cell's frame, and the 3x3 mean
+    // of Fx, Fy around an interior cell.
+    Coord GetTempCoordinate(size_t ij, double angle) const;
+    Tensor2 GetTemp2VelocityIF(size_t ij, double angle) const;
+    double GetFrequencyShift(size_t ij, double angle) const;
+    double IntensityAt(size_t ij, Tensor2 vTempIF) const;
+    Tensor2 AverageF(size_t i, size_t j) const;
+
+    Tensor3 InitialDataLFtoIF(size_t ij) const;
+    double NormalizationOverwrite() const;
 
     void LoadInitialData();
     void UpdateFourierCoefficients();

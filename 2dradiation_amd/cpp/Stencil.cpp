@@ -1,13 +1,16 @@
 #include "Stencil.h"
 #include <algorithm>
 
-// InterpolationGrid (reference src/Stencil.cpp:4-38, 109-162): for every fine
-// angle 2*pi*i/nGrid, the four stencil directions around the nearest one and
-// their cubic Lagrange weights, with angles rotated so the fine angle sits at pi.
+// InterpolationGrid (reference src/Stencil.cpp:4-38, 57-162): for every fine
+// angle 2*pi*i/nGrid, the two (linear) and four (cubic) stencil directions
+// around the nearest one and their Lagrange weights, with angles rotated so
+// the fine angle sits at pi.
 InterpolationGrid::InterpolationGrid(size_t nGrid_, const Stencil &s) : nGrid(nGrid_)
 {
     const double twoPi = 2.0 * rtm::kPi;
     const long n = (long)s.nDir;
+    neighbourIndexesLinear.resize(nGrid);
+    neighbourWeightsLinear.resize(nGrid);
     neighbourIndexesCubic.resize(nGrid);
     neighbourWeightsCubic.resize(nGrid);
     for (size_t i = 0; i < nGrid; i++) {
@@ -23,6 +26,19 @@ InterpolationGrid::InterpolationGrid(size_t nGrid_, const Stencil &s) : nGrid(nG
             }
         }
         double phiMin = std::fmod(s.Phi(index) + rtm::kPi - phi, twoPi);
+        /* linear: the nearest direction and its neighbour on the other side
+         * of the fine angle */
+        const long firstL = (phiMin < rtm::kPi) ? index : index - 1;
+        std::array<size_t, 2> nl;
+        for (int k = 0; k < 2; k++) nl[k] = (size_t)(((firstL + k) % n + n) % n);
+        std::array<double, 2> wl;
+        for (int a = 0; a < 2; a++) {
+            const double phi_a = std::fmod(s.Phi(nl[a]) + rtm::kPi - phi + twoPi, twoPi);
+            const double phi_b = std::fmod(s.Phi(nl[1 - a]) + rtm::kPi - phi + twoPi, twoPi);
+            wl[a] = (rtm::kPi - phi_b) / (phi_a - phi_b);
+        }
+        neighbourIndexesLinear[i] = nl;
+        neighbourWeightsLinear[i] = wl;
         long first = (phiMin < rtm::kPi) ? index - 1 : index - 2;
         std::array<size_t, 4> nb;
         for (int k = 0; k < 4; k++) nb[k] = (size_t)(((first + k) % n + n) % n);
@@ -204,6 +220,28 @@ void Stencil::PopulateLookUpTable()
             sigma += std::pow(10, -refinement);
         }
     }
+}
+
+void InterpolationGrid::Print() const
+{
+    std::cout << "        d,\tneighbours,\tweights,\n";
+    for (size_t d = 0; d < nGrid; d++) {
+        const auto &n = neighbourIndexesCubic[d];
+        const auto &w = neighbourWeightsCubic[d];
+        std::cout << Format(d) << ",\t(" << n[0] << "," << n[1] << "," << n[2] << "," << n[3] << ")";
+        std::cout << ",\t(" << Format(w[0]) << "," << Format(w[1]) << "," << Format(w[2]) << "," << Format(w[3])
+                  << "),\n";
+    }
+    std::cout << std::endl;
+}
+
+void Stencil::WriteToCsv() const
+{
+    std::ofstream out(OUTPUTDIR + name + ".csv");
+    out << "#d,w,phi,cx,cy,cz\n";
+    for (size_t d = 0; d < nDir; d++)
+        out << Format(d) << "," << Format(W(d)) << "," << Format(Phi(d)) << "," << Format(Cx(d)) << "," << Format(Cy(d))
+            << ",0\n";
 }
 
 void Stencil::Print() const

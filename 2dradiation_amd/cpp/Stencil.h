@@ -11,6 +11,10 @@ struct Stencil;
 
 struct InterpolationGrid {
     size_t nGrid = 0;
+    // per fine angle: the two stencil directions around it and their linear
+    // Lagrange weights, and the four around it with cubic weights
+    std::vector<std::array<size_t, 2>> neighbourIndexesLinear;
+    std::vector<std::array<double, 2>> neighbourWeightsLinear;
     std::vector<std::array<size_t, 4>> neighbourIndexesCubic;
     std::vector<std::array<double, 4>> neighbourWeightsCubic;
 
@@ -21,6 +25,8 @@ struct InterpolationGrid {
     double Cx(size_t d) const { return MyCos(Phi(d)); }
     double Cy(size_t d) const { return MySin(Phi(d)); }
     Tensor2 C(size_t d) const { return Tensor2(Cx(d), Cy(d)); }
+    // InterpolationGrid::Print (reference src/Stencil.cpp:164-173)
+    void Print() const;
 };
 
 struct Stencil {
@@ -56,6 +62,9 @@ struct Stencil {
     // Stencil::Print / PrintAll (reference src/Stencil.cpp:355-369)
     void Print() const;
     void PrintAll() const;
+    // Stencil::WriteToCsv (reference src/Stencil.cpp:370-376):
+    // OUTPUTDIR<name>.csv with "#d,w,phi,cx,cy,cz" and one line per direction
+    void WriteToCsv() const;
     void PopulateLookUpTable();
 };
 #endif

@@ -17,14 +17,21 @@ def test_driver_is_built():
     assert os.path.exists(DRIVER), "tests/cpp/_build/radiation_driver missing: run __graft_entry__.build()"
 
 
-def test_host_api_helpers():
-    """Metric values at points, frame transforms, NullNormalize, the observed
-    velocity, boosts, vector arithmetic and Logger (tests/cpp/api_check.cpp);
-    host only."""
+def test_host_api_helpers(tmp_path):
+    """Metric values and getters at cells and points, frame transforms,
+    NullNormalize, the observed velocity, boosts, vector arithmetic, Logger,
+    the linear angular tables, Grid::Print and Stencil::WriteToCsv
+    (tests/cpp/api_check.cpp); host only."""
     exe = os.path.join(ROOT, "tests", "cpp", "_build", "api_check")
     assert os.path.exists(exe), "tests/cpp/_build/api_check missing: run __graft_entry__.build()"
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    run = tmp_path / "run"
+    (tmp_path / "output").mkdir()
+    run.mkdir()
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=run)
     assert r.returncode == 0 and "api_check OK" in r.stdout, r.stdout + r.stderr
+    assert "Grid: cfl=0.5" in r.stdout
+    csv = (tmp_path / "output" / "Stencil20.4.csv").read_text().splitlines()
+    assert csv[0] == "#d,w,phi,cx,cy,cz" and len(csv) == 25
 
 
 def run_driver(mode, tmp_path, cwd=None):
