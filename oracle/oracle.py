@@ -91,6 +91,8 @@ def lib(fast=False):
     L.or_moments.argtypes = [C.POINTER(Sim), dptr, dptr]
     L.or_moments_lf.argtypes = [C.POINTER(Grid), dptr, dptr, dptr, d, dptr]
     L.or_collide.argtypes = [C.POINTER(Sim), dptr, dptr, dptr, C.c_void_p]
+    L.or_fluid_frame.argtypes = [d, d, dptr, dptr]
+    L.or_fluid_dir.argtypes = [d, d, d, d, dptr]
     L.or_myacos.restype = d
     L.or_myacos.argtypes = [d]
     L.or_stencil_ghost.restype = C.c_int

@@ -723,6 +723,41 @@ void or_moments_lf(const or_grid *g, const double *mom, const double *tetrad, co
 }
 /* Collide, Radiation.cpp:641-724.  Writes Inew at interior active cells,
  * updates the moment arrays in place, and records cycle counts. */
+/* The fluid-frame moments of a cycle (src/Radiation.cpp:664-672) from the
+ * inertial-frame moments m = (E, Fx, Fy, Pxx, Pxy, Pyy) and the fluid
+ * velocity u: out = (E_FF, Fx_FF, Fy_FF) as the reference forms them. */
+static inline void fluid_frame_moments(double u1, double u2, double lorentz, const double *m, double *out)
+{
+    double pE = m[0], pFx = m[1], pFy = m[2], pPxx = m[3], pPxy = m[4], pPyy = m[5];
+    double uF = u1 * pFx + u2 * pFy;
+    double uxP = u1 * pPxx + u2 * pPxy;
+    double uyP = u1 * pPxy + u2 * pPyy;
+    double uuP = u1 * uxP + u2 * uyP;
+    out[0] = lorentz * lorentz * (pE - 2.0 * uF + uuP);
+    out[1] = lorentz * lorentz * (pFx - lorentz * pE * u1 - uxP + u1 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
+    out[2] = lorentz * lorentz * (pFy - lorentz * pE * u2 - uyP + u2 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
+}
+/* Doppler factor A and the fluid-frame direction nFF of the inertial-frame
+ * direction (n1, n2) (src/Radiation.cpp:689-698): out = (nFF1, nFF2, A). */
+static inline void fluid_frame_dir(double u1, double u2, double lorentz, double n1, double n2, double *out)
+{
+    double un = u1 * n1 + u2 * n2;
+    double A = lorentz * (1.0 - un);
+    /* Tensor2 nFF = (nIF - (1 - (lorentz*un)/(lorentz+1)) * lorentz * uIF) / A */
+    double fac = (1 - (lorentz * un) / (lorentz + 1)) * lorentz;
+    out[0] = (n1 - fac * u1) / A;
+    out[1] = (n2 - fac * u2) / A;
+    out[2] = A;
+}
+void or_fluid_frame(double u1, double u2, const double m[6], double out[3])
+{
+    fluid_frame_moments(u1, u2, 1.0 / sqrt(1.0 - (u1 * u1 + u2 * u2)), m, out);
+}
+void or_fluid_dir(double u1, double u2, double n1, double n2, double out[3])
+{
+    fluid_frame_dir(u1, u2, 1.0 / sqrt(1.0 - (u1 * u1 + u2 * u2)), n1, n2, out);
+}
+
 /* Collide of one active cell (src/Radiation.cpp:650-724): Icell and
  * Inew_cell hold its nDir intensities; m holds E, Fx, Fy, Pxx, Pxy, Pyy of
  * Icell on entry and the moments of the last cycle on return.  Returns the
@@ -743,13 +778,9 @@ int or_collide_cell(const or_sim *s, long ij, const double *Icell, double *Inew_
     while (fabs(diff) > 1e-4 && (cycles < 100)) {
         cycles++;
         double pE = m[0], pFx = m[1], pFy = m[2], pPxx = m[3], pPxy = m[4], pPyy = m[5];
-        double uF = u1 * pFx + u2 * pFy;
-        double uxP = u1 * pPxx + u2 * pPxy;
-        double uyP = u1 * pPxy + u2 * pPyy;
-        double uuP = u1 * uxP + u2 * uyP;
-        double fE = lorentz * lorentz * (pE - 2.0 * uF + uuP);
-        double fFx = lorentz * lorentz * (pFx - lorentz * pE * u1 - uxP + u1 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
-        double fFy = lorentz * lorentz * (pFy - lorentz * pE * u2 - uyP + u2 * lorentz / (1.0 + lorentz) * ((2.0 * lorentz + 1.0) * uF - lorentz * uuP));
+        double ff[3];
+        fluid_frame_moments(u1, u2, lorentz, m, ff);
+        double fE = ff[0], fFx = ff[1], fFy = ff[2];
         double e = 0, fx = 0, fy = 0, pxx = 0, pxy = 0, pyy = 0;
         for (int d = 0; d < D; d++) {
             double n1 = s->cx[d], n2 = s->cy[d];
@@ -757,11 +788,9 @@ int or_collide_cell(const or_sim *s, long ij, const double *Icell, double *Inew_
                 n1 = s->cx[d] * rc - s->cy[d] * rs;
                 n2 = s->cx[d] * rs + s->cy[d] * rc;
             }
-            double un = u1 * n1 + u2 * n2;
-            double A = lorentz * (1.0 - un);
-            /* Tensor2 nFF = (nIF - (1 - (lorentz*un)/(lorentz+1)) * lorentz * uIF) / A */
-            double fac = (1 - (lorentz * un) / (lorentz + 1)) * lorentz;
-            double nf1 = (n1 - fac * u1) / A, nf2 = (n2 - fac * u2) / A;
+            double fd[3];
+            fluid_frame_dir(u1, u2, lorentz, n1, n2, fd);
+            double nf1 = fd[0], nf2 = fd[1], A = fd[2];
             double nF = nf1 * fFx + nf2 * fFy;
             double M = s->kappa0[ij] * fE + 3.0 * s->kappa1[ij] * nF;
             double v = (Icell[d] + alpha * g->dt * (s->eta[ij] + M) / (A * A)) /

@@ -107,6 +107,12 @@ void or_collide(const or_sim *s, const double *I, double *Inew, double *mom, int
  * are its nDir intensities, m its six moments (in: of Icell; out: after the
  * last cycle).  Returns the cycle count. */
 int or_collide_cell(const or_sim *s, long ij, const double *Icell, double *Inew_cell, double m[6]);
+/* Pieces of a cycle of Collide for fluid velocity u = (u1, u2) in the
+ * inertial frame: the fluid-frame (E, Fx, Fy) of the moments m
+ * (src/Radiation.cpp:664-672) and, for a direction n, (nFF1, nFF2, A)
+ * (:689-698). */
+void or_fluid_frame(double u1, double u2, const double m[6], double out[3]);
+void or_fluid_dir(double u1, double u2, double n1, double n2, double out[3]);
 
 /* ---- Adaptive streaming (rt_adaptive.c) ---- */
 double or_myacos(double x);                                /* src/Utility.hh:453-471 */
