@@ -74,3 +74,27 @@ def test_batch_rejects_uncovered_member(rt):
     with pytest.raises(rt.RtError) as e:
         rt.Batch([ctx])
     assert e.value.code == rt.RT_ERR_UNSUPPORTED
+
+
+@pytest.mark.gpu
+def test_full_batched_workload_matches_sequential(rt):
+    """All 64 problems of the batched workload (64^2..512^2 cells, 16..64
+    directions, Minkowski and Schwarzschild, FP32-mixed), stepped together
+    and one by one: bit-identical states."""
+    specs = rt.workloads.batch_specs(64)
+    print(f"[batch] building {len(specs)} members", flush=True)
+    A, B = [], []
+    for sp in specs:
+        A.append(rt.workloads.make_batch_member(sp, rt.FP32_MIXED)[0])
+        B.append(rt.workloads.make_batch_member(sp, rt.FP32_MIXED)[0])
+    assert 64 in {sp["nDir"] for sp in specs}
+    b = rt.Batch(A)
+    for n in (3, 4):
+        b.step(n)
+        for c in B:
+            c.step(n, want_moments=False)
+    for k, (a, c) in enumerate(zip(A, B)):
+        assert np.array_equal(a.download_I(), c.download_I()), k
+    b.close()
+    for c in A + B:
+        c.close()

@@ -90,3 +90,35 @@ def test_fp32_mixed_d64_curved_kernel_runs_fused(rt, O):
     """The headline form: FP32-mixed, 64 directions, Schwarzschild, fused."""
     gs, og, P, ctx, steps, nDir, r2 = setup(rt, O, "schwarzschild_1024sq_d64", rt.FP32_MIXED)
     assert nDir == 64 and ctx.uses_fused()
+
+
+def test_kerrschild_excision_inside_horizon_gpu_tables_vs_oracle(rt, O):
+    """The Kerr-Schild workload's tables built on the GPU (RK45 tol 1e-10,
+    cells inside r < 1.6 excised, so backward geodesics from 1.6 < r < 2
+    start inside the horizon) against the oracle's, and the FP64 step on
+    them from I = 0 against the oracle on its own tables."""
+    n, L, kind, nDir, r2, tol, steps = CASES["kerrschild_200sq_d32"]
+    gs = rt.GridSpec(n, n, (-L, -L), (L, L), 0.5)
+    og = O.make_grid(n, n, (-L, -L), (L, L), 0.5)
+    f = rt.workloads.fields("kerrschild_200sq_d32", gs)
+    M = O.metric(og, kind)
+    ref = O.geodesic_tables(og, M, r2, tol)
+    ctx = rt.Context(gs, nDir, rt.CURVED_FIXED, rt.FP64, r2)
+    w, phi, cx, cy, Nc, Wc = rt.build_stencil(nDir)
+    ctx.set_stencil(w, phi, cx, cy, Nc, Wc)
+    block, alpha, t11, t12, t21, t22 = rt.build_metric(kind, 1.0, 0.0, gs)
+    ctx.set_geometry(alpha, t11, t12, t21, t22)
+    ctx.set_streaming_stencil(*rt.build_stencil(5, tables=False))
+    ctx.build_fourier_tables(block, tol)
+    got = ctx.get_fourier_tables()
+    for a, b in zip(got, ref):
+        assert rel_l2(a, b) <= 1e-12
+    ctx.set_fields(**f)
+    ctx.upload_I(np.zeros(gs.N * nDir))
+    print(f"[exact] Kerr-Schild GPU tables {n}x{n} fused={ctx.uses_fused()}", flush=True)
+    P = O.Problem(og, nDir, 2, M, bh_r2=r2, tables=ref, fields=f)
+    Iref, mref, mx, sm, _ = P.run(np.zeros(og.N * nDir), steps)
+    ctx.reset_iteration_stats()
+    ctx.step(steps)
+    assert rel_l2(ctx.download_I(), Iref) <= TOL64
+    assert ctx.iteration_stats() == (mx, sm)
