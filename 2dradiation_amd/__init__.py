@@ -21,6 +21,7 @@ FP64, FP32_MIXED = 0, 1
 FLAT_FIXED, FLAT_ADAPTIVE, CURVED_FIXED, CURVED_ADAPTIVE, GEODESIC_FIXED = 0, 1, 2, 3, 4
 MINKOWSKI, SCHWARZSCHILD, KERRSCHILD = 0, 1, 2
 NMETRIC = 29
+NOALLOC_SCRATCH_DOUBLES = 1 << 17  # RT_NOALLOC_SCRATCH_DOUBLES
 
 
 class RtError(RuntimeError):
@@ -91,6 +92,7 @@ def lib():
     L.rt_get_iteration_counts.argtypes = [vp, _ip]
     L.rt_forward_checkpointed.argtypes = [vp, i32, i32]
     L.rt_sample_outgoing.argtypes = [vp, i32, _dp, _dp, _dp]
+    L.rt_sample_outgoing_noalloc.argtypes = [vp, i32, _dp, _dp, _dp]
     L.rt_adjoint.argtypes = [vp, i32, _dp, _dp, _dp, _dp, _dp]
     L.rt_batch_create.argtypes = [C.POINTER(vp), i32, C.POINTER(vp)]
     L.rt_batch_destroy.argtypes = [vp]
@@ -358,6 +360,15 @@ class Context:
         y = np.ascontiguousarray(y, dtype=np.float64)
         out = np.zeros(len(x) * self.nDir)
         self._c(lib().rt_sample_outgoing(self.h, len(x), x, y, out))
+        return out.reshape(len(x), self.nDir)
+
+    def sample_outgoing_noalloc(self, x, y):
+        """sample_outgoing staged in the library's fixed per-GPU array (no
+        device allocation); 2*npts + npts*nDir <= NOALLOC_SCRATCH_DOUBLES."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros(len(x) * self.nDir)
+        self._c(lib().rt_sample_outgoing_noalloc(self.h, len(x), x, y, out))
         return out.reshape(len(x), self.nDir)
 
     def adjoint(self, x, y, dL_dout):

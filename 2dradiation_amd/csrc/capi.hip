@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 #include "rt_internal.h"
 #include "../cpp/Spacetimes.h"
@@ -15,8 +16,39 @@ int launch_mask(rt_ctx *c);
 static int ensure_unscaled(rt_ctx *c);
 static int build_tables_on_device(rt_ctx *c, double tol);
 
+/* Staging of rt_sample_outgoing_noalloc: the sample points and their
+ * samples, RT_NOALLOC_SCRATCH_DOUBLES doubles on every GPU the library runs
+ * on, for the life of the process.  Host code uses each device's address of
+ * it (g_scratch), never the name: the name is a host-side stand-in. */
+static __device__ __align__(256) double scratch[RT_NOALLOC_SCRATCH_DOUBLES];
+
 namespace {
 thread_local std::string g_create_error;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+/* The address of `scratch` on the current device (d), looked up once per
+ * device. */
+int scratch_address(int d, double **out, std::string &err)
+{
+    if (d < 0 || d >= kMaxDevices) {
+        err = "device ordinal beyond the no-allocation table";
+        return RT_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[d]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return RT_ERR_HIP;
+        }
+        g_scratch[d] = (double *)p;
+    }
+    *out = g_scratch[d];
+    return RT_OK;
+}
 
 template <typename P>
 int dalloc(rt_ctx *c, P **p, size_t bytes)
@@ -248,6 +280,10 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
         rt_destroy(c);
         return rc;
     };
+    {
+        std::string serr;
+        if (int rs = scratch_address(d.device, &c->d_scratch, serr)) return fail(rt_fail(c, rs, serr));
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(rt_fail(c, RT_ERR_HIP, "stream creation failed"));
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(rt_fail(c, RT_ERR_HIP, "event creation failed"));
     const size_t N = c->N, D = c->D;
@@ -1001,6 +1037,24 @@ int rt_sample_outgoing(rt_ctx *c, int32_t npts, const double *x, const double *y
     TRY(upload_points(c, npts, x, y, nullptr));
     double *dout = c->d_pts + 2 * npts;
     TRY(launch_sample(c, c->d_I[c->cur], c->scaled ? 1 : 0, npts, c->d_pts, c->d_pts + npts, dout));
+    RT_HIP(c, hipMemcpyAsync(out, dout, (size_t)npts * c->D * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_sample_outgoing_noalloc(rt_ctx *c, int32_t npts, const double *x, const double *y, double *out)
+{
+    if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
+    NEED(c, x); NEED(c, y); NEED(c, out);
+    TRY(check_points(c, npts, x, y));
+    if (2 * (long long)npts + (long long)npts * c->D > RT_NOALLOC_SCRATCH_DOUBLES)
+        return rt_fail(c, RT_ERR_ARG, "rt_sample_outgoing_noalloc: 2*npts + npts*nDir exceeds the " +
+                                          std::to_string(RT_NOALLOC_SCRATCH_DOUBLES) + " doubles of its staging array");
+    if (!c->d_scratch) return rt_fail(c, RT_ERR_STATE, "no-allocation staging array not set up");
+    double *px = c->d_scratch, *py = px + npts, *dout = py + npts;
+    RT_HIP(c, hipMemcpyAsync(px, x, npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(py, y, npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TRY(launch_sample(c, c->d_I[c->cur], c->scaled ? 1 : 0, npts, px, py, dout));
     RT_HIP(c, hipMemcpyAsync(out, dout, (size_t)npts * c->D * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     RT_HIP(c, hipStreamSynchronize(c->stream));
     return RT_OK;

@@ -150,6 +150,9 @@ struct rt_ctx {
     double *d_grad[2] = {nullptr, nullptr};
     double *d_pts = nullptr; /* sample points and their gradients */
     int pts_cap = 0;
+    /* this device's address of the no-allocation staging array
+     * (rt_sample_outgoing_noalloc) */
+    double *d_scratch = nullptr;
 };
 
 int rt_fail(rt_ctx *c, int code, const std::string &msg);

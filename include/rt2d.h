@@ -238,6 +238,17 @@ int rt_forward_checkpointed(rt_ctx *ctx, int32_t nSteps, int32_t every);
  * (n_d . (x, y) > 0) and 0 for the others (an observer ring around the
  * origin). */
 int rt_sample_outgoing(rt_ctx *ctx, int32_t npts, const double *x, const double *y, double *out);
+/* rt_sample_outgoing without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * points and samples are staged in one fixed device array per GPU of
+ * RT_NOALLOC_SCRATCH_DOUBLES doubles (1 MiB, held until the process ends),
+ * so 2*npts + npts*nDir must not exceed it (RT_ERR_ARG otherwise).  Every
+ * context on a GPU shares that array: calls that overlap on one GPU race
+ * (calls in sequence, the usual case, do not: each returns after its
+ * samples are back on the host).  Same outputs as rt_sample_outgoing. */
+#define RT_NOALLOC_SCRATCH_DOUBLES (1 << 17)
+int rt_sample_outgoing_noalloc(rt_ctx *ctx, int32_t npts, const double *x, const double *y, double *out);
 /* Gradient of a loss L(samples) of the state after the last
  * rt_forward_checkpointed run: dL_dout [npts*nDir] is dL/d(out) of
  * rt_sample_outgoing at the same points.  Recomputes each checkpoint segment
