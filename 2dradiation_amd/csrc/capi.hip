@@ -287,9 +287,11 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(rt_fail(c, RT_ERR_HIP, "stream creation failed"));
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(rt_fail(c, RT_ERR_HIP, "event creation failed"));
     const size_t N = c->N, D = c->D;
-    if ((r = dalloc(c, &c->d_I[0], N * D * tsize(c))) || (r = dalloc(c, &c->d_I[1], N * D * tsize(c))) ||
+    /* the state buffers carry a 256-byte tail guard: the fused kernel's
+     * 16-byte row staging may read up to 2 elements past the last row */
+    if ((r = dalloc(c, &c->d_I[0], N * D * tsize(c) + 256)) || (r = dalloc(c, &c->d_I[1], N * D * tsize(c) + 256)) ||
         (r = dalloc(c, &c->d_mom, 6 * N * sizeof(double))) || (r = dalloc(c, &c->d_count, N * sizeof(int32_t))) ||
-        (r = dalloc(c, &c->d_stats, 2 * sizeof(unsigned long long))) || (r = dalloc(c, &c->d_mask, N)) ||
+        (r = dalloc(c, &c->d_stats, RT_STAT_SLOTS * RT_STAT_STRIDE * sizeof(unsigned long long))) || (r = dalloc(c, &c->d_mask, N)) ||
         (r = dalloc(c, &c->d_f64, 6 * N * sizeof(double))) || (r = dalloc(c, &c->d_cell, RT_CELL_FIELDS * N * tsize(c))) ||
         (r = dalloc(c, &c->d_alpha, N * sizeof(double))) || (r = dalloc(c, &c->d_t11, N * sizeof(double))) ||
         (r = dalloc(c, &c->d_t12, N * sizeof(double))) || (r = dalloc(c, &c->d_t21, N * sizeof(double))) ||
@@ -323,7 +325,7 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
     if (hipMemsetAsync(c->d_I[0], 0, N * D * tsize(c), c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_I[1], 0, N * D * tsize(c), c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_mom, 0, 6 * N * sizeof(double), c->stream) != hipSuccess ||
-        hipMemsetAsync(c->d_stats, 0, 2 * sizeof(unsigned long long), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_stats, 0, RT_STAT_SLOTS * RT_STAT_STRIDE * sizeof(unsigned long long), c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_count, 0, N * sizeof(int32_t), c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_f64, 0, 6 * N * sizeof(double), c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_t12, 0, N * sizeof(double), c->stream) != hipSuccess ||
@@ -717,11 +719,16 @@ int rt_get_moments_lf(rt_ctx *c, double *E_LF, double *Fx_LF, double *Fy_LF, dou
 int rt_iteration_stats(rt_ctx *c, int32_t *maxCount, double *sumCount)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
-    unsigned long long s[2];
+    unsigned long long s[RT_STAT_SLOTS * RT_STAT_STRIDE];
     RT_HIP(c, hipMemcpyAsync(s, c->d_stats, sizeof s, hipMemcpyDeviceToHost, c->stream));
     RT_HIP(c, hipStreamSynchronize(c->stream));
-    if (maxCount) *maxCount = (int32_t)s[1];
-    if (sumCount) *sumCount = (double)s[0];
+    unsigned long long sum = 0, mx = 0;
+    for (int k = 0; k < RT_STAT_SLOTS; k++) {
+        sum += s[k * RT_STAT_STRIDE];
+        mx = std::max(mx, s[k * RT_STAT_STRIDE + 1]);
+    }
+    if (maxCount) *maxCount = (int32_t)mx;
+    if (sumCount) *sumCount = (double)sum;
     return RT_OK;
 }
 
@@ -737,7 +744,7 @@ int rt_get_iteration_counts(rt_ctx *c, int32_t *count)
 int rt_reset_iteration_stats(rt_ctx *c)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
-    RT_HIP(c, hipMemsetAsync(c->d_stats, 0, 2 * sizeof(unsigned long long), c->stream));
+    RT_HIP(c, hipMemsetAsync(c->d_stats, 0, RT_STAT_SLOTS * RT_STAT_STRIDE * sizeof(unsigned long long), c->stream));
     RT_HIP(c, hipStreamSynchronize(c->stream));
     return RT_OK;
 }

@@ -41,6 +41,15 @@ __device__ inline T tmax0(T v)
  * constant address space lets the compiler use scalar loads. */
 typedef const __attribute__((address_space(4))) DirTables CDirTables;
 
+/* One 16-byte direct global-to-LDS load per lane (gfx950): lane l writes
+ * dst + 16 l; dst must be wave-uniform and 16-byte aligned, src 16-byte
+ * aligned.  (A non-template function: inside a template the host pass
+ * rejects the 16-byte form and drops the kernel's host stub.) */
+__device__ inline void lds_dma16(const void *src, void *dst)
+{
+    __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+}
+
 /* Copies n dwords from global src to LDS dst with one dword per lane per
  * instruction; lanes past n do nothing.  dst must be wave-uniform. */
 __device__ inline void lds_dma_dwords(const void *src, void *dst, int n, int lane)

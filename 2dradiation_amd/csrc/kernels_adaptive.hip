@@ -620,8 +620,9 @@ __global__ __launch_bounds__(256) void k_collide_adaptive(const T *__restrict__ 
             mx = mx > o ? mx : o;
         }
         if (lane == 0 && mx > 0) {
-            atomicAdd(&stats[0], sum);
-            atomicMax(&stats[1], mx);
+            unsigned long long *st = rt_stat_slot(stats, blockIdx.x);
+            atomicAdd(&st[0], sum);
+            atomicMax(&st[1], mx);
         }
     }
     __syncthreads();

@@ -201,8 +201,9 @@ __global__ __launch_bounds__(128) void k_collide_exact(CollideArgs a)
         m = m > o ? m : o;
     }
     if ((threadIdx.x & 63) == 0 && m > 0) {
-        atomicAdd(&a.stats[0], s);
-        atomicMax(&a.stats[1], m);
+        unsigned long long *st = rt_stat_slot(a.stats, (unsigned)(ij >> 6));
+        atomicAdd(&st[0], s);
+        atomicMax(&st[1], m);
     }
 }
 

@@ -60,6 +60,7 @@ struct KP {
     int strips, chunks, rows_per_block, nblocks;
     double startx, starty, dx, dy, dtime, r2;
     int want_moments;
+    int wide; /* FP32 16-byte staging (Smem) allowed for this launch */
     /* adjoint recording (k_fused<..., REC = true>): moments of the streamed
      * intensities [6][N], lambda cycle counts [N], corner clamp bits [4][N] */
     double *rec_m;
@@ -102,21 +103,38 @@ __device__ __noinline__ bool zero_start_stops(const double *__restrict__ f64, co
     return e == 0.0 || fx == 0.0 || fy == 0.0 || pxx == 0.0 || pxy == 0.0 || pyy == 0.0;
 }
 
-/* LDS of one workgroup.  Intensity rows are SoA [direction][66 columns]
+/* LDS of one workgroup.  Intensity rows are SoA [direction][columns]
  * holding directions -2..D+3 (mod D), so the 5 angular taps of a corner are
  * 5 rows of one column at constant offsets, never wrapped; lanes
- * (= columns) hit consecutive banks.  Rows and FP64 coefficients arrive by
- * direct global-to-LDS loads, without passing through registers. */
+ * (= columns) hit consecutive banks.  A row covers the strip's 64 cells and
+ * one halo cell each side (66 columns).  Rows and per-cell inputs arrive by
+ * direct global-to-LDS loads, without passing through registers.
+ *
+ * FP32 rows and per-cell rows are 68 wide, a whole number of 16-byte
+ * chunks: on grids of even width (and N a multiple of 4) they are staged
+ * with 16-byte loads from the aligned column at or left of the first one
+ * needed, 0 or 2 columns (rows) and 1 or 3 (cells) ahead of it; the readers
+ * add that shift.  Otherwise 4-byte loads fill them from column 0. */
 template <typename T, int D, bool CURVED, int NW>
-struct Smem {
-    static constexpr int RW = 66;
+struct __attribute__((aligned(16))) Smem {
+    static constexpr int RL = 66;                        /* columns used */
+    static constexpr int RW = sizeof(T) == 4 ? 68 : 66;  /* row stride */
+    static constexpr int CW = sizeof(T) == 4 ? 68 : 64;  /* per-cell row stride */
+    /* the deferred collide of the tile body (its CP) in the forms that can
+     * hold a row of streamed intensities in registers */
+    static constexpr bool HAS_CP = NW == 4 && (sizeof(T) == 8 || D >= 48);
     T I[3][RT_RING_ROWS(D)][RW];
-    T C[CURVED ? 25 : 1][64];                    /* coefficient series (d_coefT form) */
-    unsigned long long EX[CURVED ? 64 : 1];      /* excised-departure bits */
-    T F[RT_CELL_FIELDS][64];                     /* per-cell fields */
+    T C[CURVED ? 25 : 1][CW];                    /* coefficient series (d_coefT form) */
+    unsigned long long EX[CURVED ? (sizeof(T) == 4 ? 66 : 64) : 2]; /* excised-departure bits */
+    T F[RT_CELL_FIELDS][CW];                     /* per-cell fields */
     double X[NW][6][64];                         /* per-wavefront moment partials */
+    T CP[HAS_CP ? 4 : 1][64];                    /* collide of a row: q - 1, b, c Fx, c Fy */
     float wrow[10][12] __attribute__((aligned(16)));
 };
+/* two workgroups per CU for the FP32 64-direction curved form, three for 32
+ * directions (512-byte granules) */
+static_assert((sizeof(Smem<float, 64, true, 4>) + 511) / 512 * 2 <= 320, "FP32 D=64 curved LDS exceeds half a CU");
+static_assert((sizeof(Smem<float, 32, true, 4>) + 511) / 512 * 3 <= 320, "FP32 D=32 curved LDS exceeds a third of a CU");
 
 /* Eight-wavefront workgroups (D/8 directions each) for FP32 with 32 or 48
  * directions and FP64 with 32, 48 or 64.  The launch takes them when the
@@ -236,7 +254,9 @@ __device__ inline void collide_moving(T (&sv)[DW], double (&mOut)[6], int &cycle
 template <typename T, int D, bool CURVED, int NW>
 constexpr int fused_waves_per_simd()
 {
-    constexpr int by_lds = (int)((160 * 1024) / sizeof(Smem<T, D, CURVED, NW>));
+    /* LDS is allocated to workgroups in 512-byte granules */
+    constexpr size_t lds = (sizeof(Smem<T, D, CURVED, NW>) + 511) / 512 * 512;
+    constexpr int by_lds = (int)((160 * 1024) / lds);
     constexpr int wgs = by_lds < 1 ? 1 : (by_lds > 3 ? 3 : by_lds);
     return wgs * NW / 4 > 4 ? 4 : wgs * NW / 4;
 }
@@ -325,6 +345,13 @@ KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, con
     p.rec_mask = rec ? rec->mask : nullptr;
     p.f64 = c->d_f64;
     p.alpha64 = c->d_alpha;
+    /* 16-byte staging reads up to 2 columns past a row's last needed one:
+     * only the state buffers carry the tail guard that keeps this in bounds */
+    auto a16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
+    p.wide = (sizeof(T) == 4 && !rec && (in == c->d_I[0] || in == c->d_I[1]) && d.nx % 2 == 0 && p.N % 4 == 0 &&
+              a16(in) && a16(c->d_cell) && a16(c->d_coefT) && a16(c->d_excl))
+                 ? 1
+                 : 0;
     return p;
 }
 

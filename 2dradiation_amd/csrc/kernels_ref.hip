@@ -298,8 +298,9 @@ __global__ void k_ref_collide(const T *__restrict__ I, T *__restrict__ Inew, dou
     }
     const int cycles = ref_collide_cell<T>(ij, I, Inew, mom, f, alpha, t11, t12, t21, t22, w, cx, cy, N, D, dt, rot);
     count[ij] = cycles;
-    atomicAdd(&stats[0], (unsigned long long)cycles);
-    atomicMax(&stats[1], (unsigned long long)cycles);
+    unsigned long long *st = rt_stat_slot(stats, (unsigned)ij);
+    atomicAdd(&st[0], (unsigned long long)cycles);
+    atomicMax(&st[1], (unsigned long long)cycles);
 }
 
 /* ------------------------------------------------------------------------

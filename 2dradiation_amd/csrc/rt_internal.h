@@ -11,6 +11,17 @@
 
 #define RT_MAX_FUSED_DIR 64
 #define RT_FUSED_STRIP 64 /* cells per wavefront row segment */
+/* Lambda-cycle statistics: RT_STAT_SLOTS (sum, max) pairs RT_STAT_STRIDE
+ * words (one 128-byte line) apart.  Each wavefront adds its partial to the
+ * pair its key selects, so the atomics of one launch spread over many
+ * lines instead of queueing on one; the host reduces the pairs. */
+#define RT_STAT_SLOTS 64
+#define RT_STAT_STRIDE 16
+__device__ inline unsigned long long *rt_stat_slot(unsigned long long *stats, unsigned key)
+{
+    return stats + (size_t)(key & (RT_STAT_SLOTS - 1)) * RT_STAT_STRIDE;
+}
+
 #define RT_FUSED_WAVES 4
 /* LDS intensity rows of one grid row: directions -2..D+3 (mod D), so that
  * every 5-tap angular window [s0, s0+4] with s0 in [-2, D-1] (and the window
@@ -129,7 +140,7 @@ struct rt_ctx {
     double *d_src_rot = nullptr;
     bool src_rot_set = false;
     int32_t *d_count = nullptr;   /* [N] per-method collide cycle counts */
-    unsigned long long *d_stats = nullptr; /* [2] sum, max */
+    unsigned long long *d_stats = nullptr; /* RT_STAT_SLOTS (sum, max) pairs */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     /* adjoint: checkpoints of the last rt_forward_checkpointed run (states
