@@ -304,6 +304,9 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
     if (d.streaming == RT_CURVED_FIXED && ((r = dalloc(c, &c->d_coefT, 5 * (size_t)d.nCoeff * N * tsize(c))) ||
                                            (r = dalloc(c, &c->d_excl, N * sizeof(unsigned long long)))))
         return fail(r);
+    if (d.streaming == RT_CURVED_FIXED && c->fp32 && D <= RT_MAX_FUSED_DIR &&
+        (r = dalloc(c, &c->d_arec, N * D * sizeof(uint32_t))))
+        return fail(r);
     /* FP64 fixed streaming: the streamed intensities between the fused
      * stream and the exact collide */
     if (!c->fp32 && !rt_is_adaptive(d) && (r = dalloc(c, &c->d_J, N * D * sizeof(double)))) return fail(r);
@@ -345,7 +348,7 @@ void rt_destroy(rt_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_w); dfree(c->d_phi); dfree(c->d_cx); dfree(c->d_cy); dfree(c->d_Nc); dfree(c->d_Wc);
     dfree(c->d_basis); dfree(c->d_dirt); dfree(c->d_alpha); dfree(c->d_t11); dfree(c->d_t12);
-    dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_coefT); dfree(c->d_excl); dfree(c->d_red);
+    dfree(c->d_t21); dfree(c->d_t22); dfree(c->d_coef); dfree(c->d_coefT); dfree(c->d_arec); dfree(c->d_excl); dfree(c->d_red);
     dfree(c->d_frame); dfree(c->d_lf); dfree(c->d_src_idx); dfree(c->d_src_I); dfree(c->d_metric);
     dfree(c->d_rot[0]); dfree(c->d_rot[1]); dfree(c->d_rtrig); dfree(c->d_src_rot);
     dfree(c->d_rtrigf); dfree(c->d_Wcf);

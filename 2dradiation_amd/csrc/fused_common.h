@@ -144,6 +144,58 @@ __device__ inline void geometry_from_series(T s, T di, T dj, T vx, T vy, CDirTab
     G.s0 = q10 + woff + RT_RING_PAD;
 }
 
+/* Angular window of a departure velocity (vx, vy) in FP64, as
+ * geometry_from_series<double> forms it: first LDS row s0 of the 5 combined
+ * taps (window offset and ring pad included), weight-row residue r and
+ * fraction t. */
+__device__ inline void angle_window64(double vx, double vy, CDirTables *dt, int &s0, int &r, double &t)
+{
+    const double q1 = double(dt->quarter);
+    double dd = angle_index_bf<double>(vx, vy, dt->ang_u, q1, 2.0 * q1, 3.0 * q1, 4.0 * q1);
+    dd = fmin(fmax(dd, 1e-8), double(dt->nGrid) - 1e-8);
+    int g0 = (int)dd;
+    t = dd - double(g0);
+    g0 = min(max(g0, 0), dt->nGrid);
+    const int q10 = g0 / 10;
+    r = g0 - 10 * q10;
+    s0 = q10 + dt->woff[r] + RT_RING_PAD;
+}
+
+/* Angular record of one (cell, direction): s0 in bits 25..31, r in 21..24,
+ * t in 21 fixed-point bits (rounded, at most 1 - 2^-21). */
+__device__ inline uint32_t pack_angle_rec(int s0, int r, double t)
+{
+    const double tq = fmin(rint(t * 2097152.0), 2097151.0);
+    return ((uint32_t)s0 << 25) | ((uint32_t)r << 21) | (uint32_t)tq;
+}
+
+/* FP32 geometry from the S, X and Y series values and the angular record
+ * (see geometry_from_series for the departure part).  s0 and r are clamped
+ * into the ring and the weight table for any record. */
+template <int D>
+__device__ inline void geometry_from_record(float s, float di, float dj, uint32_t rec, const float *wrow,
+                                            DirGeom<float> &G)
+{
+    G.s = s;
+    const float fi = fminf(fmaxf(floorf(di), -1.0f), 0.0f);
+    const float fj = fminf(fmaxf(floorf(dj), -1.0f), 0.0f);
+    G.fx = di - fi;
+    G.fy = dj - fj;
+    G.lo_i = fi < 0.0f;
+    G.lo_j = fj < 0.0f;
+    const int r = min((int)((rec >> 21) & 15u), 9);
+    const float t = (float)(rec & 0x1FFFFFu) * 0x1p-21f;
+    const float4 *wr = (const float4 *)((const char *)wrow + __umul24((unsigned)r, 48u));
+    const float4 w0 = wr[0], w1 = wr[1];
+    const float2 w2 = *(const float2 *)(wr + 2);
+    G.Cw[0] = w0.x + t * w1.y;
+    G.Cw[1] = w0.y + t * w1.z;
+    G.Cw[2] = w0.z + t * w1.w;
+    G.Cw[3] = w0.w + t * w2.x;
+    G.Cw[4] = w1.x + t * w2.y;
+    G.s0 = min((int)(rec >> 25), D + 1);
+}
+
 template <typename T, int D>
 __device__ inline void curved_geometry(const T (&cS)[5], const T (&cX)[5], const T (&cY)[5], const T (&cU)[5],
                                        const T (&cV)[5], CDirTables *dtab, CDirTables *dtab0, const float *wrow,

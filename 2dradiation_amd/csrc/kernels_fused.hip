@@ -52,6 +52,7 @@ struct KP {
     const double *coef;         /* [25][N] FP64 */
     const T *coefT;             /* [25][N] in kernel form (S - 1, relative X, Y) */
     const unsigned long long *excl; /* [N] excised-departure bits */
+    const uint32_t *arec;       /* FP32 curved: [D][N] angular records (k_angle_rec) */
     double *mom;                /* [6][N] */
     unsigned long long *stats;  /* [2] */
     const DirTables *dt;
@@ -123,8 +124,11 @@ struct __attribute__((aligned(16))) Smem {
     /* the deferred collide of the tile body (its CP) in the forms that can
      * hold a row of streamed intensities in registers */
     static constexpr bool HAS_CP = NW == 4 && (sizeof(T) == 8 || D >= 48);
+    /* series staged per cell: FP32 takes its angular windows from the
+     * records (k_angle_rec) and evaluates only S, X and Y */
+    static constexpr int NSER = sizeof(T) == 4 ? 15 : 25;
     T I[3][RT_RING_ROWS(D)][RW];
-    T C[CURVED ? 25 : 1][CW];                    /* coefficient series (d_coefT form) */
+    T C[CURVED ? NSER : 1][CW];                  /* coefficient series (d_coefT form) */
     unsigned long long EX[CURVED ? (sizeof(T) == 4 ? 66 : 64) : 2]; /* excised-departure bits */
     T F[RT_CELL_FIELDS][CW];                     /* per-cell fields */
     double X[NW][6][64];                         /* per-wavefront moment partials */
@@ -316,6 +320,7 @@ KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, con
     p.coef = c->d_coef;
     p.coefT = (const T *)c->d_coefT;
     p.excl = c->d_excl;
+    p.arec = c->d_arec;
     p.mom = c->d_mom;
     p.stats = c->d_stats;
     p.dt = c->d_dirt;
@@ -490,6 +495,37 @@ __global__ void k_coef_rel(const double *__restrict__ coef, T *__restrict__ out,
     }
 }
 
+/* Angular record of every (cell, direction) of an FP32 fixed curved context:
+ * the departure velocity series (15..24 of the FP64 tables) at phi_d and the
+ * window the FP64 fused stream takes for it (angle_window64), in the same
+ * arithmetic, packed by pack_angle_rec.  The FP32 stream then evaluates only
+ * the S, X and Y series. */
+__global__ void k_angle_rec(const double *__restrict__ coef, const DirTables *__restrict__ dt0, uint32_t *__restrict__ rec,
+                            long N, int D)
+{
+    const long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (ij >= N) return;
+    CDirTables *dt = (CDirTables *)dt0;
+    double cU[5], cV[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        cU[k] = coef[(size_t)(15 + k) * N + ij];
+        cV[k] = coef[(size_t)(20 + k) * N + ij];
+    }
+    for (int d = 0; d < D; d++) {
+        double vx = cU[0], vy = cV[0];
+#pragma unroll
+        for (int q = 1; q < 5; q++) {
+            vx += cU[q] * dt->basis[d][q];
+            vy += cV[q] * dt->basis[d][q];
+        }
+        int s0, r;
+        double t;
+        angle_window64(vx, vy, dt, s0, r, t);
+        rec[(size_t)d * N + ij] = pack_angle_rec(s0, r, t);
+    }
+}
+
 int launch_coef_rel(rt_ctx *c)
 {
     if (!rt_is_curved(c->desc) || c->nCoeff != 5 || !c->d_coefT) return RT_OK;
@@ -502,6 +538,10 @@ int launch_coef_rel(rt_ctx *c)
         k_coef_rel<double><<<nb, 256, 0, c->stream>>>(c->d_coef, (double *)c->d_coefT, d.nx, c->N, d.startx, d.starty,
                                                       d.dx, d.dy);
     RT_HIP(c, hipGetLastError());
+    if (c->d_arec && c->has_stencil) {
+        k_angle_rec<<<nb, 256, 0, c->stream>>>(c->d_coef, c->d_dirt, c->d_arec, (long)c->N, c->D);
+        RT_HIP(c, hipGetLastError());
+    }
     return RT_OK;
 }
 

@@ -86,6 +86,9 @@ struct rt_ctx {
     /* the same coefficients in the fused kernel's form and precision: S - 1,
      * X and Y relative to the cell centre in cells, velocity as is */
     void *d_coefT = nullptr;
+    /* FP32 fixed curved streaming: angular window of every (cell, direction)
+     * from the FP64 series, [D][N] (k_angle_rec) */
+    uint32_t *d_arec = nullptr;
     uint8_t *d_mask = nullptr;
     double max_disp = 0;          /* max |departure - cell| in cells (curved tables) */
     /* bit d set: the departure point of direction d (< 64) is excised */

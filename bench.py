@@ -34,7 +34,10 @@ def algorithmic_bytes_per_update(nDir, b_i, b_f, curved=True, nCoeff=5):
     alpha + 5 series * nCoeff coefficients in the kernel's precision for
     curved spacetimes: the FP64 step reads the FP64 tables, the FP32-mixed
     kernel an FP32 copy of them in relative form (S - 1, departure offsets in
-    cells), so b_geo is 8 (1 + 25) = 208 B or 4 (1 + 25) = 104 B per cell."""
+    cells), so b_geo is 8 (1 + 25) = 208 B or 4 (1 + 25) = 104 B per cell.
+    Derived per-(cell, direction) tables are not algorithmic: the FP32
+    kernel's 4-byte angular record per update (which stands in for the two
+    velocity series) counts against the achieved fraction, in `traffic`."""
     b_geo = b_f * (1 + 5 * nCoeff) if curved else 0
     return 2 * b_i + (6 * b_f + b_geo) / nDir
 
