@@ -304,9 +304,12 @@ int rt_create(const rt_desc *desc, rt_ctx **out)
     if (d.streaming == RT_CURVED_FIXED && ((r = dalloc(c, &c->d_coefT, 5 * (size_t)d.nCoeff * N * tsize(c))) ||
                                            (r = dalloc(c, &c->d_excl, N * sizeof(unsigned long long)))))
         return fail(r);
-    if (d.streaming == RT_CURVED_FIXED && c->fp32 && D <= RT_MAX_FUSED_DIR &&
-        (r = dalloc(c, &c->d_arec, N * D * sizeof(uint32_t))))
-        return fail(r);
+    if (d.streaming == RT_CURVED_FIXED && c->fp32 && D <= RT_MAX_FUSED_DIR) {
+        if ((r = dalloc(c, &c->d_arec, N * D * sizeof(uint32_t)))) return fail(r);
+        /* zero records are valid windows (the fused kernel reads them unchecked) */
+        if (hipMemsetAsync(c->d_arec, 0, N * D * sizeof(uint32_t), c->stream) != hipSuccess)
+            return fail(rt_fail(c, RT_ERR_HIP, "record clear failed"));
+    }
     /* FP64 fixed streaming: the streamed intensities between the fused
      * stream and the exact collide */
     if (!c->fp32 && !rt_is_adaptive(d) && (r = dalloc(c, &c->d_J, N * D * sizeof(double)))) return fail(r);

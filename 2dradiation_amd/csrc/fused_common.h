@@ -170,8 +170,9 @@ __device__ inline uint32_t pack_angle_rec(int s0, int r, double t)
 }
 
 /* FP32 geometry from the S, X and Y series values and the angular record
- * (see geometry_from_series for the departure part).  s0 and r are clamped
- * into the ring and the weight table for any record. */
+ * (see geometry_from_series for the departure part).  Records are valid by
+ * construction (k_angle_rec, or zero before the first tables): s0 <= D + 1
+ * keeps the taps in the ring and r <= 9 in the weight table. */
 template <int D>
 __device__ inline void geometry_from_record(float s, float di, float dj, uint32_t rec, const float *wrow,
                                             DirGeom<float> &G)
@@ -183,7 +184,7 @@ __device__ inline void geometry_from_record(float s, float di, float dj, uint32_
     G.fy = dj - fj;
     G.lo_i = fi < 0.0f;
     G.lo_j = fj < 0.0f;
-    const int r = min((int)((rec >> 21) & 15u), 9);
+    const int r = (int)((rec >> 21) & 15u);
     const float t = (float)(rec & 0x1FFFFFu) * 0x1p-21f;
     const float4 *wr = (const float4 *)((const char *)wrow + __umul24((unsigned)r, 48u));
     const float4 w0 = wr[0], w1 = wr[1];
@@ -193,7 +194,7 @@ __device__ inline void geometry_from_record(float s, float di, float dj, uint32_
     G.Cw[2] = w0.z + t * w1.w;
     G.Cw[3] = w0.w + t * w2.x;
     G.Cw[4] = w1.x + t * w2.y;
-    G.s0 = min((int)(rec >> 25), D + 1);
+    G.s0 = (int)(rec >> 25);
 }
 
 template <typename T, int D>
