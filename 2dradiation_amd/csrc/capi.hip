@@ -359,6 +359,9 @@ void rt_destroy(rt_ctx *c)
     dfree(c->d_lam[0]); dfree(c->d_lam[1]); dfree(c->d_grad[0]); dfree(c->d_grad[1]); dfree(c->d_pts); dfree(c->d_mask); dfree(c->d_f64);
     dfree(c->d_cell); dfree(c->d_I[0]); dfree(c->d_I[1]); dfree(c->d_J); dfree(c->d_mom); dfree(c->d_count);
     dfree(c->d_stats);
+    for (hipEvent_t e : {c->ev_seg[0], c->ev_seg[1], c->ev_adj[0], c->ev_adj[1], c->ev_go})
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -990,7 +993,7 @@ int rt_forward_checkpointed(rt_ctx *c, int32_t nSteps, int32_t every)
     if (count > c->ck_cap) {
         dfree(c->d_ckpt);
         c->ck_cap = 0;
-        TRY(dalloc(c, &c->d_ckpt, (size_t)count * bytes));
+        TRY(dalloc(c, &c->d_ckpt, (size_t)count * bytes + 256)); /* tail guard, as d_I */
         c->ck_cap = count;
     }
     TRY(prepare_fused(c));
@@ -1087,11 +1090,18 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
     if (every > c->seg_cap) {
         dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
         c->seg_cap = 0;
-        TRY(dalloc(c, &c->d_seg, (size_t)every * bytes));
-        TRY(dalloc(c, &c->d_rec_m, (size_t)every * 6 * N * sizeof(double)));
-        TRY(dalloc(c, &c->d_rec_k, (size_t)every * N));
-        TRY(dalloc(c, &c->d_rec_mask, (size_t)every * 8 * N * sizeof(unsigned long long)));
+        /* two segments: states (tail guard, as d_I) and records */
+        TRY(dalloc(c, &c->d_seg, 2 * (size_t)every * bytes + 256));
+        TRY(dalloc(c, &c->d_rec_m, 2 * (size_t)every * 6 * N * sizeof(double)));
+        TRY(dalloc(c, &c->d_rec_k, 2 * (size_t)every * N));
+        TRY(dalloc(c, &c->d_rec_mask, 2 * (size_t)every * 8 * N * sizeof(unsigned long long)));
         c->seg_cap = every;
+    }
+    if (!c->stream2) {
+        /* default priority, like `stream` */
+        RT_HIP(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        for (hipEvent_t *e : {&c->ev_seg[0], &c->ev_seg[1], &c->ev_adj[0], &c->ev_adj[1], &c->ev_go})
+            RT_HIP(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
     TRY(dalloc(c, &c->d_lam[0], bytes));
     TRY(dalloc(c, &c->d_lam[1], bytes));
@@ -1114,36 +1124,63 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
     /* dL/d(final state) from the sampled outputs */
     TRY(launch_sample_adjoint(c, c->d_lam[lc], c->ck_scaled ? 1 : 0, npts, c->d_pts, c->d_pts + npts,
                               c->d_pts + 2 * npts));
-    auto rec_at = [&](int i) {
+    /* segment buffer b (0, 1): the state after step i and its records */
+    auto seg_at = [&](int b, int i) { return (char *)c->d_seg + ((size_t)b * every + i) * bytes; };
+    auto rec_at = [&](int b, int i) {
+        const size_t k = (size_t)b * every + i;
         RecordPtrs r;
-        r.m = c->d_rec_m + (size_t)i * 6 * N;
-        r.k = c->d_rec_k + (size_t)i * N;
-        r.mask = c->d_rec_mask + (size_t)i * 8 * N;
+        r.m = c->d_rec_m + k * 6 * N;
+        r.k = c->d_rec_k + k * N;
+        r.mask = c->d_rec_mask + k * 8 * N;
         return r;
     };
     /* halo and excised cells keep their values in every state: copy them
      * into the segment buffers once */
-    for (int i = 0; i < std::min(every, c->ck_steps); i++)
-        TRY(launch_copy_fixed_cells(c, c->d_ckpt, (char *)c->d_seg + (size_t)i * bytes));
+    for (int b = 0; b < 2; b++)
+        for (int i = 0; i < std::min(every, c->ck_steps); i++)
+            TRY(launch_copy_fixed_cells(c, c->d_ckpt, seg_at(b, i)));
     /* the adjoint step clears the active cells of the buffer it reads, which
      * then takes the next step's scatter; what accumulates in fixed cells is
      * never read (fixed cells do not depend on active ones) */
     RT_HIP(c, hipMemsetAsync(c->d_lam[1 - lc], 0, bytes, c->stream));
-    for (int s = c->ck_count - 1; s >= 0; s--) {
+    /* recompute segment s into buffer b on stream2, keeping every state
+     * after a step */
+    auto recompute = [&](int s, int b) -> int {
         const int steps = std::min(every, c->ck_steps - s * every);
-        /* recompute the segment, keeping every state after a step */
         const void *prev = (const char *)c->d_ckpt + (size_t)s * bytes;
-        for (int i = 0; i < steps; i++) {
-            void *out = (char *)c->d_seg + (size_t)i * bytes;
-            TRY(launch_fused_step_rec(c, prev, out, rec_at(i)));
-            prev = out;
+        hipStream_t keep = c->stream;
+        c->stream = c->stream2;
+        int r = RT_OK;
+        for (int i = 0; i < steps && r == RT_OK; i++) {
+            r = launch_fused_step_rec(c, prev, seg_at(b, i), rec_at(b, i));
+            prev = seg_at(b, i);
         }
+        c->stream = keep;
+        return r;
+    };
+    /* stream2 starts after the set-up above */
+    RT_HIP(c, hipEventRecord(c->ev_go, c->stream));
+    RT_HIP(c, hipStreamWaitEvent(c->stream2, c->ev_go, 0));
+    const int last = c->ck_count - 1;
+    if (last >= 0) {
+        TRY(recompute(last, 0));
+        RT_HIP(c, hipEventRecord(c->ev_seg[0], c->stream2));
+    }
+    for (int s = last, b = 0; s >= 0; s--, b ^= 1) {
+        if (s >= 1) {
+            /* buffer 1-b last served segment s+1's adjoint steps */
+            if (s < last) RT_HIP(c, hipStreamWaitEvent(c->stream2, c->ev_adj[1 - b], 0));
+            TRY(recompute(s - 1, 1 - b));
+            RT_HIP(c, hipEventRecord(c->ev_seg[1 - b], c->stream2));
+        }
+        RT_HIP(c, hipStreamWaitEvent(c->stream, c->ev_seg[b], 0));
         /* adjoint steps, last to first */
+        const int steps = std::min(every, c->ck_steps - s * every);
         for (int i = steps - 1; i >= 0; i--) {
-            TRY(launch_adjoint_step(c, c->d_lam[lc], c->d_lam[1 - lc], (const char *)c->d_seg + (size_t)i * bytes,
-                                    rec_at(i)));
+            TRY(launch_adjoint_step(c, c->d_lam[lc], c->d_lam[1 - lc], seg_at(b, i), rec_at(b, i)));
             lc = 1 - lc;
         }
+        RT_HIP(c, hipEventRecord(c->ev_adj[b], c->stream));
     }
     RT_HIP(c, hipMemcpyAsync(grad_eta, c->d_grad[0], N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     RT_HIP(c, hipMemcpyAsync(grad_kappa0, c->d_grad[1], N * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -42,6 +42,7 @@ struct AP {
     const uint8_t *mask;
     const T *coefT;
     const unsigned long long *excl;
+    const uint32_t *arec; /* FP32 curved: angular records of the forward (k_angle_rec) */
     const DirTables *dt;
     double *grad_eta, *grad_k0;
     int nx, ny;
@@ -107,12 +108,14 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
             const int ncell = min(64, nx - 2 - col0);
             const size_t off = (size_t)row * nx + col0 + 1;
             constexpr int E = (int)(sizeof(T) / 4);
+            /* FP32 takes its windows from the records: S, X and Y only */
+            constexpr int NS = sizeof(T) == 4 ? 15 : 25;
 #pragma unroll 1
-            for (int e = wv; e < 26; e += RT_FUSED_WAVES) {
-                const uint32_t *src = e < 25 ? (const uint32_t *)(p.coefT + (size_t)e * N + off)
+            for (int e = wv; e < NS + 1; e += RT_FUSED_WAVES) {
+                const uint32_t *src = e < NS ? (const uint32_t *)(p.coefT + (size_t)e * N + off)
                                              : (const uint32_t *)(p.excl + off);
-                uint32_t *dst = e < 25 ? (uint32_t *)&sm.C[e][0] : (uint32_t *)&sm.EX[0];
-                const int n = (e < 25 ? E : 2) * ncell;
+                uint32_t *dst = e < NS ? (uint32_t *)&sm.C[e][0] : (uint32_t *)&sm.EX[0];
+                const int n = (e < NS ? E : 2) * ncell;
                 if (ncell == 64) {
                     /* full strip: whole unmasked instructions */
                     for (int b = 0; b < n; b += 64) __builtin_amdgcn_global_load_lds(src + b + lane, dst + b, 4, 0, 0);
@@ -125,17 +128,28 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
     /* adds the finished accumulation row `row` into the global adjoint and
      * clears its slot */
     auto flush = [&](int row) {
-        double *slot = &sm.A[row % 3][0][0];
-        for (int e = threadIdx.x; e < NA; e += 256) {
-            const int r = e / RW, cidx = e - r * RW;
-            const double v = slot[e];
-            const int col = col0 + cidx;
-            if (v != 0.0 && col < nx) {
-                int d = r - RT_RING_PAD;
-                d = d < 0 ? d + D : (d >= D ? d - D : d);
-                unsafeAtomicAdd(p.lam_out + (size_t)d * N + (size_t)row * nx + col, (T)v);
+        /* wavefront = direction, lane = column (64 lanes, then the two
+         * right-hand columns); the ring rows that repeat a direction (pad
+         * rows 0, 1 and D+2..D+5) are folded into its row first, so every
+         * element takes one global atomic per workgroup */
+        const int ncol = min(RW, nx - col0);
+#pragma unroll 1
+        for (int d = wv; d < D; d += RT_FUSED_WAVES) {
+            const int rp = d + RT_RING_PAD;
+            const int dup = d >= D - 2 ? rp - D : (d < 4 ? rp + D : -1);
+            T *dst = p.lam_out + (size_t)d * N + (size_t)row * nx + col0;
+            double *src = &sm.A[row % 3][rp][0];
+            double *srd = &sm.A[row % 3][dup < 0 ? rp : dup][0];
+#pragma unroll
+            for (int c0 = 0; c0 < RW; c0 += 64) {
+                const int c = c0 + lane;
+                if (c < RW) {
+                    const double v = src[c] + (dup < 0 ? 0.0 : srd[c]);
+                    src[c] = 0.0;
+                    srd[c] = 0.0;
+                    if (v != 0.0 && c < ncol) unsafeAtomicAdd(dst + c, (T)v);
+                }
             }
-            slot[e] = 0.0;
         }
     };
     stage_coef(rowA);
@@ -145,6 +159,14 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
         __syncthreads(); /* coefficients of row j staged; accumulation slot of row j+1 clear */
         const bool more = (j + 1 < rowB);
         const unsigned ij = (unsigned)j * (unsigned)nx + (unsigned)gi;
+        /* FP32 curved: the forward's angular windows of this row, read
+         * first (used by the scatter at the end of the row) */
+        constexpr bool AREC = CURVED && sizeof(T) == 4;
+        uint32_t arec[AREC ? DW : 1];
+        if constexpr (AREC) {
+#pragma unroll
+            for (int k = 0; k < DW; k++) arec[k] = inside ? (p.arec + (size_t)(wv * DW + k) * N)[ij] : 0u;
+        }
         const uint8_t mk = inside ? p.mask[ij] : 0;
         const bool active = (mk & MASK_ACTIVE) != 0;
         CDirTables *dtab = dtab0;
@@ -254,8 +276,8 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
                 cS[k] = sm.C[k][lane];
                 cX[k] = sm.C[5 + k][lane];
                 cY[k] = sm.C[10 + k][lane];
-                cU[k] = sm.C[15 + k][lane];
-                cV[k] = sm.C[20 + k][lane];
+                cU[k] = AREC ? T(0) : sm.C[15 + k][lane];
+                cV[k] = AREC ? T(0) : sm.C[20 + k][lane];
             }
             const uint32_t exw = (uint32_t)(sm.EX[lane] >> (wv * DW));
             unsigned long long mbits = 0ull;
@@ -263,17 +285,19 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
                 if (p.rec_waves == 8) {
                     /* recording wavefronts 2 wv and 2 wv + 1 hold this
                      * wavefront's first and second DW/2 directions */
-                    const unsigned long long lo = (p.rec_mask + (size_t)(2 * wv) * N)[ij];
-                    const unsigned long long hi = (p.rec_mask + (size_t)(2 * wv + 1) * N)[ij];
+                    const unsigned long long lo = D <= 32 ? ((const uint32_t *)p.rec_mask + (size_t)(2 * wv) * N)[ij]
+                                                          : (p.rec_mask + (size_t)(2 * wv) * N)[ij];
+                    const unsigned long long hi = D <= 32 ? ((const uint32_t *)p.rec_mask + (size_t)(2 * wv + 1) * N)[ij]
+                                                          : (p.rec_mask + (size_t)(2 * wv + 1) * N)[ij];
                     mbits = lo | (hi << (2 * DW));
                 } else {
-                    mbits = (p.rec_mask + (size_t)wv * N)[ij];
+                    mbits = D <= 32 ? ((const uint32_t *)p.rec_mask + (size_t)wv * N)[ij] : (p.rec_mask + (size_t)wv * N)[ij];
                 }
             }
             const T angu = (sizeof(T) == 4) ? (T)dtab->ang_uf : (T)dtab->ang_u;
             const int quarter = dtab->quarter, nGrid = dtab->nGrid;
             const T ddmax = T(nGrid) - T(1e-8);
-            f32x4 ser[5];
+            f32x4 ser[3];
 #pragma unroll
             for (int k = 0; k < DW; k++) {
                 const int d = wv * DW + k;
@@ -282,8 +306,6 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
                         ser[0] = series4_mfma(cS, bas4[k / 4]);
                         ser[1] = series4_mfma(cX, bas4[k / 4]);
                         ser[2] = series4_mfma(cY, bas4[k / 4]);
-                        ser[3] = series4_mfma(cU, bas4[k / 4]);
-                        ser[4] = series4_mfma(cV, bas4[k / 4]);
                     }
                 }
                 const double jb = (double)lam[k] + a3 * dtab->w6[d][0] *
@@ -293,8 +315,7 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
                     DirGeom<T> G;
                     if constexpr (sizeof(T) == 4) {
                         const int i = k % 4;
-                        geometry_from_series<T, D>(ser[0][i], ser[1][i], ser[2][i], ser[3][i], ser[4][i], dtab0,
-                                                   &sm.wrow[0][0], angu, quarter, nGrid, ddmax, G);
+                        geometry_from_record<D>(ser[0][i], ser[1][i], ser[2][i], arec[k], &sm.wrow[0][0], G);
                     } else {
                         curved_geometry<T, D>(cS, cX, cY, cU, cV, dtab, dtab0, &sm.wrow[0][0], d, angu, quarter,
                                               nGrid, ddmax, G);
@@ -416,6 +437,7 @@ int launch_adj(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, 
     p.mask = c->d_mask;
     p.coefT = (const T *)c->d_coefT;
     p.excl = c->d_excl;
+    p.arec = c->d_arec;
     p.dt = c->d_dirt;
     p.grad_eta = c->d_grad[0];
     p.grad_k0 = c->d_grad[1];
@@ -435,10 +457,12 @@ int launch_adj(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, 
     const int wgs = per_cu * c->ncu;
     int chunks = (wgs + strips - 1) / strips;
     int H = (rows + chunks - 1) / chunks;
-    /* chunks down to 2 rows: on small grids that fills a second resident
-     * workgroup per CU where the LDS allows it (FP32 256^2 x 32: 40 against
-     * 51 ms for 500 adjoint steps), at the price of more halo-row atomics */
-    H = H < 2 ? 2 : (H > 64 ? 64 : H);
+    /* chunks down to 3 rows: on small grids that fills more of the second
+     * resident workgroup per CU the LDS allows, at the price of more
+     * halo-row atomics (every element of a chunk's H + 2 completed rows takes
+     * one global atomic; FP32 256^2 x 32, 500 steps with their recomputation:
+     * 2 rows 32.4 ms, 3 rows 30.9, 4 rows 36.9) */
+    H = H < 3 ? 3 : (H > 64 ? 64 : H);
     chunks = (rows + H - 1) / H;
     p.chunks = chunks;
     p.rows_per_block = H;

@@ -78,7 +78,7 @@ __global__ __launch_bounds__(128) void k_collide_exact(CollideArgs a)
         if (REC) {
             const double m0[6] = {E, Fx, Fy, Pxx, Pxy, Pyy};
 #pragma unroll
-            for (int X = 0; X < 6; X++) a.rec_m[(size_t)X * N + ij] = m0[X];
+            for (int X = 0; X < 3; X++) a.rec_m[(size_t)X * N + ij] = m0[X]; /* E, Fx, Fy: what the adjoint reads */
         }
 
         const double uu = u1 * u1 + u2 * u2;

@@ -62,8 +62,10 @@ struct KP {
     double startx, starty, dx, dy, dtime, r2;
     int want_moments;
     int wide; /* FP32 16-byte staging (Smem) allowed for this launch */
-    /* adjoint recording (k_fused<..., REC = true>): moments of the streamed
-     * intensities [6][N], lambda cycle counts [N], corner clamp bits [4][N] */
+    /* adjoint recording (k_fused<..., REC = true>): moments E, Fx, Fy of the
+     * streamed intensities [3][N] (of a [6][N] buffer), lambda cycle counts
+     * [N], corner clamp bits [4 or 8][N] (32-bit words up to 32 directions,
+     * else 64-bit) */
     double *rec_m;
     uint8_t *rec_k;
     unsigned long long *rec_mask;
@@ -351,10 +353,12 @@ KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, con
     p.f64 = c->d_f64;
     p.alpha64 = c->d_alpha;
     /* 16-byte staging reads up to 2 columns past a row's last needed one:
-     * only the state buffers carry the tail guard that keeps this in bounds */
+     * every buffer the fused step reads a state from (d_I, and the
+     * checkpoints and segment states of the adjoint) carries a 256-byte tail
+     * guard, or is followed by another state of the same allocation */
     auto a16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
-    p.wide = (sizeof(T) == 4 && !rec && (in == c->d_I[0] || in == c->d_I[1]) && d.nx % 2 == 0 && p.N % 4 == 0 &&
-              a16(in) && a16(c->d_cell) && a16(c->d_coefT) && a16(c->d_excl))
+    p.wide = (sizeof(T) == 4 && d.nx % 2 == 0 && p.N % 4 == 0 && a16(in) && a16(c->d_cell) && a16(c->d_coefT) &&
+              a16(c->d_excl))
                  ? 1
                  : 0;
     return p;

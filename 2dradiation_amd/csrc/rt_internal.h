@@ -159,6 +159,10 @@ struct rt_ctx {
     double *d_rec_m = nullptr;
     uint8_t *d_rec_k = nullptr;
     unsigned long long *d_rec_mask = nullptr; /* [every][8][N]: clamp bits per recording wavefront */
+    /* the adjoint recomputes segment s-1 on stream2 while segment s's adjoint
+     * steps run on `stream`: d_seg and the records hold two segments */
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_seg[2] = {nullptr, nullptr}, ev_adj[2] = {nullptr, nullptr}, ev_go = nullptr;
     int rec_waves = RT_FUSED_WAVES;             /* wavefronts of the last recording launch (4 or 8) */
     void *d_lam[2] = {nullptr, nullptr};
     double *d_grad[2] = {nullptr, nullptr};
