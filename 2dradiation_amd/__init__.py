@@ -102,6 +102,7 @@ def lib():
     L.rt_batch_time_steps.argtypes = [vp, i32, C.POINTER(C.c_float)]
     L.rt_batch_autotune.argtypes = [vp, i32, vp]
     L.rt_batch_groups.argtypes = [vp, C.POINTER(i32)]
+    L.rt_batch_uses_queue.argtypes = [vp, C.POINTER(i32)]
     L.rt_build_stencil.argtypes = [i32, _dp, _dp, _dp, _dp, vp, vp]
     L.rt_build_stencil_ghost.argtypes = [i32, i32, _dp, _dp, _dp, _dp, vp, vp]
     L.rt_set_rotation.argtypes = [vp, _dp, vp]
@@ -418,6 +419,12 @@ class Batch:
         ms = C.c_float()
         self._c(lib().rt_batch_time_steps(self.h, n, C.byref(ms)))
         return ms.value
+
+    def uses_queue(self):
+        """True when the FP32 members step in one persistent queue launch."""
+        q = C.c_int32()
+        self._c(lib().rt_batch_uses_queue(self.h, C.byref(q)))
+        return bool(q.value)
 
     def autotune(self, trial_steps=2):
         """Rows per workgroup chosen per class (0 = default rule)."""

@@ -62,6 +62,7 @@ struct KP {
     double startx, starty, dx, dy, dtime, r2;
     int want_moments;
     int wide; /* FP32 16-byte staging (Smem) allowed for this launch */
+    int cls;  /* k_fused_queue: index of (nDir, streaming) in queue_class */
     /* adjoint recording (k_fused<..., REC = true>): moments E, Fx, Fy of the
      * streamed intensities [3][N] (of a [6][N] buffer), lambda cycle counts
      * [N], corner clamp bits [4 or 8][N] (32-bit words up to 32 directions,
@@ -275,9 +276,22 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks)
     return (nblocks & 7) == 0 ? (bid & 7) * (nblocks >> 3) + (bid >> 3) : bid;
 }
 
+/* One tile of the fused step (fused_tile_body.inc) in the workgroup's LDS
+ * sm; PK: KP<T> in the constant address space, so that its fields are reloaded
+ * where used rather than held in scalar registers across the tile. */
+template <typename T, int D, bool CURVED, bool REC, int NW, bool SONLY, typename PK>
+__device__ __forceinline__ void fused_tile(PK &p, int tile, const T *Iin, T *Iout, Smem<T, D, CURVED, NW> &sm)
+{
+#include "fused_tile_body.inc"
+}
+
+/* The single-problem kernel includes the body itself: its parameters stay
+ * in the kernel-argument segment, which the compiler reloads freely (through
+ * a reference they would take scalar registers for the whole body). */
 template <typename T, int D, bool CURVED, bool REC, int NW, bool SONLY>
 __global__ __launch_bounds__(64 * NW, (fused_waves_per_simd<T, D, CURVED, NW>())) void k_fused(KP<T> p)
 {
+    __shared__ Smem<T, D, CURVED, NW> sm;
     const int tile = xcd_tile(blockIdx.x, p.nblocks);
     const T *Iin = p.Iin;
     T *Iout = p.Iout;
@@ -292,8 +306,7 @@ template <typename T, int D, bool CURVED>
 __global__ __launch_bounds__(64 * RT_FUSED_WAVES, (fused_waves_per_simd<T, D, CURVED, RT_FUSED_WAVES>())) void k_fused_batch(
     const KP<T> *__restrict__ probs, const int *__restrict__ start, int nprob, int parity)
 {
-    constexpr bool REC = false, SONLY = false;
-    constexpr int NW = RT_FUSED_WAVES;
+    __shared__ Smem<T, D, CURVED, RT_FUSED_WAVES> sm;
     int i = 0;
     while (i + 1 < nprob && start[i + 1] <= (int)blockIdx.x) i++;
     i = __builtin_amdgcn_readfirstlane(i);
@@ -301,7 +314,77 @@ __global__ __launch_bounds__(64 * RT_FUSED_WAVES, (fused_waves_per_simd<T, D, CU
     const int tile = xcd_tile((int)blockIdx.x - start[i], p.nblocks);
     const T *Iin = parity ? (const T *)p.Iout : p.Iin;
     T *Iout = parity ? const_cast<T *>(p.Iin) : p.Iout;
+    constexpr bool REC = false, SONLY = false;
+    constexpr int NW = RT_FUSED_WAVES;
 #include "fused_tile_body.inc"
+}
+
+/* ---- one persistent launch for a batch of FP32 problems ------------------
+ * Every FP32 problem of a batch, whatever its nDir and streaming, in one
+ * launch of resident workgroups that take (problem, tile) items from a queue
+ * (largest tiles first) until it is empty.  Class c = 2 * (index of nDir in
+ * 16, 20, 24, 32, 48, 64) + curved selects the tile body; the LDS is the
+ * union of the classes' layouts. */
+union QueueSmem {
+    Smem<float, 16, false, RT_FUSED_WAVES> f16;
+    Smem<float, 16, true, RT_FUSED_WAVES> c16;
+    Smem<float, 20, false, RT_FUSED_WAVES> f20;
+    Smem<float, 20, true, RT_FUSED_WAVES> c20;
+    Smem<float, 24, false, RT_FUSED_WAVES> f24;
+    Smem<float, 24, true, RT_FUSED_WAVES> c24;
+    Smem<float, 32, false, RT_FUSED_WAVES> f32;
+    Smem<float, 32, true, RT_FUSED_WAVES> c32;
+    Smem<float, 48, false, RT_FUSED_WAVES> f48;
+    Smem<float, 48, true, RT_FUSED_WAVES> c48;
+    Smem<float, 64, false, RT_FUSED_WAVES> f64;
+    Smem<float, 64, true, RT_FUSED_WAVES> c64;
+};
+
+int queue_class(int D, bool curved)
+{
+    const int k = D == 16 ? 0 : D == 20 ? 1 : D == 24 ? 2 : D == 32 ? 3 : D == 48 ? 4 : D == 64 ? 5 : -1;
+    return k < 0 ? -1 : 2 * k + (curved ? 1 : 0);
+}
+
+/* Item w = (problem, tile) of the table; *counter starts at 0 (k_queue_reset). */
+__global__ __launch_bounds__(64 * RT_FUSED_WAVES, 2) void k_fused_queue(const KP<float> *__restrict__ probs,
+                                                                        const int2 *__restrict__ items, int nitems,
+                                                                        int *counter, int parity)
+{
+    __shared__ QueueSmem sm;
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads(); /* the previous tile is done with the LDS and with s_item */
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= nitems) return;
+        const int2 w = items[it];
+        typedef const __attribute__((address_space(4))) KP<float> CKP;
+        CKP &p = ((CKP *)probs)[w.x];
+        const float *Iin = parity ? (const float *)p.Iout : p.Iin;
+        float *Iout = parity ? const_cast<float *>(p.Iin) : p.Iout;
+        constexpr int NW = RT_FUSED_WAVES;
+        switch (p.cls) {
+        case 0: fused_tile<float, 16, false, false, NW, false>(p, w.y, Iin, Iout, sm.f16); break;
+        case 1: fused_tile<float, 16, true, false, NW, false>(p, w.y, Iin, Iout, sm.c16); break;
+        case 2: fused_tile<float, 20, false, false, NW, false>(p, w.y, Iin, Iout, sm.f20); break;
+        case 3: fused_tile<float, 20, true, false, NW, false>(p, w.y, Iin, Iout, sm.c20); break;
+        case 4: fused_tile<float, 24, false, false, NW, false>(p, w.y, Iin, Iout, sm.f24); break;
+        case 5: fused_tile<float, 24, true, false, NW, false>(p, w.y, Iin, Iout, sm.c24); break;
+        case 6: fused_tile<float, 32, false, false, NW, false>(p, w.y, Iin, Iout, sm.f32); break;
+        case 7: fused_tile<float, 32, true, false, NW, false>(p, w.y, Iin, Iout, sm.c32); break;
+        case 8: fused_tile<float, 48, false, false, NW, false>(p, w.y, Iin, Iout, sm.f48); break;
+        case 9: fused_tile<float, 48, true, false, NW, false>(p, w.y, Iin, Iout, sm.c48); break;
+        case 10: fused_tile<float, 64, false, false, NW, false>(p, w.y, Iin, Iout, sm.f64); break;
+        default: fused_tile<float, 64, true, false, NW, false>(p, w.y, Iin, Iout, sm.c64); break;
+        }
+    }
+}
+
+__global__ void k_queue_reset(int *counter)
+{
+    if (threadIdx.x == 0) counter[0] = 0;
 }
 
 /* Kernel parameters of one fused step of context c; H = rows per workgroup
@@ -569,6 +652,18 @@ struct rt_batch {
     std::vector<Group> groups;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    /* one persistent launch for all FP32 members (k_fused_queue) instead of
+     * one launch per FP32 group; qH: rows per tile (0: the single-problem
+     * rule; 64 problems of 64^2..512^2 x 16..64 directions take 191 ms per
+     * 200 steps with it, 151 ms with 16 or 32 rows, which the autotuner
+     * picks) */
+    bool queue = true;
+    int qH = 16;
+    int q_nitems = 0, q_grid = 0;
+    KP<float> *d_qkp = nullptr;
+    int2 *d_items = nullptr;
+    int *d_counter = nullptr;
+    size_t q_items_cap = 0;
 };
 
 namespace {
@@ -602,6 +697,86 @@ int fill_group_table(rt_batch *b, rt_batch::Group &g)
     BHIP(b, hipMemcpyAsync(g.d_kp, kp.data(), kp.size() * sizeof(KP<T>), hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipMemcpyAsync(g.d_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
+    return RT_OK;
+}
+
+/* Resident workgroups of k_fused_queue on the batch's device. */
+int queue_grid(const rt_batch *b)
+{
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_queue, 64 * RT_FUSED_WAVES, 0) != hipSuccess || n < 1)
+            n = 1;
+        per_cu = n;
+    }
+    return per_cu * b->ctx[0]->ncu;
+}
+
+bool has_fp32(const rt_batch *b)
+{
+    for (const auto &g : b->groups)
+        if (g.fp32) return true;
+    return false;
+}
+
+/* The problem table and the item queue of k_fused_queue: every FP32 member
+ * with b->qH rows per tile (0: the single-problem default for a round of
+ * resident workgroups), tiles ordered by decreasing work (rows x nDir, curved
+ * counted twice), ties by problem and tile. */
+int fill_queue_table(rt_batch *b)
+{
+    if (!b->queue || !has_fp32(b)) return RT_OK;
+    b->q_grid = queue_grid(b);
+    std::vector<KP<float>> kp;
+    struct Item {
+        double cost;
+        int prob, tile;
+    };
+    std::vector<Item> items;
+    for (const auto &g : b->groups) {
+        if (!g.fp32) continue;
+        for (int m : g.members) {
+            rt_ctx *c = b->ctx[m];
+            KP<float> p = make_kp<float>(c, c->d_I[c->cur], c->d_I[1 - c->cur], false, nullptr, b->qH, b->q_grid);
+            p.cls = queue_class(c->D, g.curved);
+            if (p.cls < 0) return bfail(b, RT_ERR_UNSUPPORTED, "batch: unsupported nDir");
+            const int prob = (int)kp.size();
+            for (int t = 0; t < p.nblocks; t++) {
+                const int chunk = t % p.chunks;
+                const int rowA = 1 + chunk * p.rows_per_block;
+                const int rows = std::min(rowA + p.rows_per_block, p.ny - 1) - rowA;
+                items.push_back({(double)rows * c->D * (g.curved ? 2.0 : 1.0), prob, t});
+            }
+            kp.push_back(p);
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
+    std::vector<int2> w(items.size());
+    for (size_t k = 0; k < items.size(); k++) w[k] = make_int2(items[k].prob, items[k].tile);
+    b->q_nitems = (int)w.size();
+    if (!b->d_qkp) BHIP(b, hipMalloc((void **)&b->d_qkp, kp.size() * sizeof(KP<float>)));
+    if (w.size() > b->q_items_cap) {
+        if (b->d_items) (void)hipFree(b->d_items);
+        b->d_items = nullptr;
+        BHIP(b, hipMalloc((void **)&b->d_items, w.size() * sizeof(int2)));
+        b->q_items_cap = w.size();
+    }
+    if (!b->d_counter) BHIP(b, hipMalloc((void **)&b->d_counter, sizeof(int)));
+    BHIP(b, hipMemcpyAsync(b->d_qkp, kp.data(), kp.size() * sizeof(KP<float>), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->d_items, w.data(), w.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return RT_OK;
+}
+
+/* One step of every FP32 member in one persistent launch. */
+int launch_queue(rt_batch *b, int parity)
+{
+    k_queue_reset<<<1, 64, 0, b->stream>>>(b->d_counter);
+    BHIP(b, hipGetLastError());
+    k_fused_queue<<<b->q_grid, 64 * RT_FUSED_WAVES, 0, b->stream>>>(b->d_qkp, b->d_items, b->q_nitems, b->d_counter,
+                                                                   parity);
+    BHIP(b, hipGetLastError());
     return RT_OK;
 }
 
@@ -645,11 +820,16 @@ int launch_group(rt_batch *b, const rt_batch::Group &g, int parity)
     return launch_group_exact(b, g, parity);
 }
 
-/* One step of every problem (one launch per group). */
+/* One step of every problem: the FP32 members in one queue launch (or one
+ * launch per FP32 group), the FP64 members as rt_step does. */
 int launch_all(rt_batch *b, int parity)
 {
+    const bool q = b->queue && has_fp32(b);
+    if (q)
+        if (int r = launch_queue(b, parity)) return r;
     for (const auto &g : b->groups)
-        if (int r = launch_group(b, g, parity)) return r;
+        if (!(q && g.fp32))
+            if (int r = launch_group(b, g, parity)) return r;
     return RT_OK;
 }
 
@@ -665,7 +845,7 @@ int batch_prepare(rt_batch *b)
     }
     for (auto &g : b->groups)
         if (int r = g.fp32 ? fill_group_table<float>(b, g) : fill_group_table<double>(b, g)) return r;
-    return RT_OK;
+    return fill_queue_table(b);
 }
 
 void drop_graph(rt_batch *b)
@@ -713,6 +893,7 @@ int batch_run(rt_batch *b, int nSteps)
             if (int r = g.fp32 ? fill_group_table<float>(b, g) : fill_group_table<double>(b, g)) return r;
             exact = exact || !g.fp32;
         }
+        if (int r = fill_queue_table(b)) return r;
         if (exact)
             if (int r = capture_graph(b)) return r;
     }
@@ -776,6 +957,9 @@ int rt_batch_destroy(rt_batch *b)
         if (g.d_kp) (void)hipFree(g.d_kp);
         if (g.d_start) (void)hipFree(g.d_start);
     }
+    if (b->d_qkp) (void)hipFree(b->d_qkp);
+    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_counter) (void)hipFree(b->d_counter);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -832,6 +1016,37 @@ int rt_batch_autotune(rt_batch *b, int32_t trial_steps, int32_t *chosen)
         }
     }
     static const int candidates[] = {0, 1, 2, 4, 8, 16, 32, 64};
+    /* times 2*trial_steps steps of `launch` after one untimed one */
+    auto time_it = [&](auto launch, float &ms) -> int {
+        int r = launch(0);
+        if (r == RT_OK && hipEventRecord(b->ev0, b->stream) != hipSuccess) r = RT_ERR_HIP;
+        for (int k = 0; r == RT_OK && k < 2 * trial_steps; k++) r = launch((k + 1) & 1);
+        if (r == RT_OK && hipEventRecord(b->ev1, b->stream) != hipSuccess) r = RT_ERR_HIP;
+        if (r == RT_OK && hipEventSynchronize(b->ev1) != hipSuccess) r = RT_ERR_HIP;
+        if (r == RT_OK && hipEventElapsedTime(&ms, b->ev0, b->ev1) != hipSuccess) r = RT_ERR_HIP;
+        return r;
+    };
+    /* the FP32 members in one queue launch: rows per tile */
+    float best_q = 1e30f, sum_groups = 0;
+    int best_qH = 0;
+    if (has_fp32(b)) {
+        b->queue = true;
+        for (int H : candidates) {
+            if (H == 1) continue; /* one-row tiles stage their halo rows twice over */
+            b->qH = H;
+            float ms = 0;
+            int r = fill_queue_table(b);
+            if (r == RT_OK) r = time_it([&](int par) { return launch_queue(b, par); }, ms);
+            if (r != RT_OK) {
+                release();
+                return r == RT_ERR_HIP ? bfail(b, r, "autotune: timing failed") : r;
+            }
+            if (ms < best_q) {
+                best_q = ms;
+                best_qH = H;
+            }
+        }
+    }
     int gi = 0;
     for (auto &g : b->groups) {
         float best = 1e30f;
@@ -840,13 +1055,7 @@ int rt_batch_autotune(rt_batch *b, int32_t trial_steps, int32_t *chosen)
             for (auto &h : g.H) h = H;
             int r = g.fp32 ? fill_group_table<float>(b, g) : fill_group_table<double>(b, g);
             float ms = 0;
-            /* one untimed step, then an even number of timed ones */
-            if (r == RT_OK) r = launch_group(b, g, 0);
-            if (r == RT_OK && hipEventRecord(b->ev0, b->stream) != hipSuccess) r = RT_ERR_HIP;
-            for (int k = 0; r == RT_OK && k < 2 * trial_steps; k++) r = launch_group(b, g, (k + 1) & 1);
-            if (r == RT_OK && hipEventRecord(b->ev1, b->stream) != hipSuccess) r = RT_ERR_HIP;
-            if (r == RT_OK && hipEventSynchronize(b->ev1) != hipSuccess) r = RT_ERR_HIP;
-            if (r == RT_OK && hipEventElapsedTime(&ms, b->ev0, b->ev1) != hipSuccess) r = RT_ERR_HIP;
+            if (r == RT_OK) r = time_it([&](int par) { return launch_group(b, g, par); }, ms);
             if (r != RT_OK) {
                 release();
                 return r == RT_ERR_HIP ? bfail(b, r, "autotune: timing failed") : r;
@@ -858,7 +1067,18 @@ int rt_batch_autotune(rt_batch *b, int32_t trial_steps, int32_t *chosen)
         }
         for (auto &h : g.H) h = bestH;
         if (chosen) chosen[gi] = bestH;
+        if (g.fp32) sum_groups += best;
         gi++;
+    }
+    /* the queue when it beats one launch per FP32 group */
+    b->queue = has_fp32(b) && best_q <= sum_groups;
+    b->qH = best_qH;
+    if (b->queue && chosen) {
+        gi = 0;
+        for (auto &g : b->groups) {
+            if (g.fp32) chosen[gi] = best_qH;
+            gi++;
+        }
     }
     for (size_t i = 0; i < b->ctx.size(); i++) {
         rt_ctx *c = b->ctx[i];
@@ -872,6 +1092,13 @@ int rt_batch_autotune(rt_batch *b, int32_t trial_steps, int32_t *chosen)
     release();
     if (int r = batch_prepare(b)) return r;
     return capture_graph(b);
+}
+
+int rt_batch_uses_queue(const rt_batch *b, int32_t *queue)
+{
+    if (!b || !queue) return RT_ERR_ARG;
+    *queue = (b->queue && has_fp32(b)) ? 1 : 0;
+    return RT_OK;
 }
 
 int rt_batch_groups(const rt_batch *b, int32_t *ngroups)

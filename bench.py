@@ -197,9 +197,10 @@ def adjoint_leg(rt, steps=500, every=50, n_fd=16, h=1e-3):
 def batch_leg(rt, steps=200, n=64):
     """64 independent FP32-mixed problems (nx = ny log-uniform in [64, 512],
     nDir from {16, 24, 32, 48, 64}, Minkowski or Schwarzschild by a seeded coin),
-    200 steps each: one graph-replayed grouped launch per class per step
-    (autotuned rows per workgroup) against the 64 problems stepped one after
-    another with their own launches.  Both runs must agree bit for bit."""
+    200 steps each, graph-replayed: one persistent launch draining a (problem,
+    tile) queue per step, or one grouped launch per class, as the autotuner
+    finds faster (with its rows per tile) against the 64 problems stepped one
+    after another with their own launches.  Both runs must agree bit for bit."""
     specs = rt.workloads.batch_specs(n)
     A, B, upd = [], [], 0
     for sp in specs:
@@ -228,6 +229,7 @@ def batch_leg(rt, steps=200, n=64):
            "sequential_updates_per_s": upd * (steps - 1) / (ms_seq * 1e-3),
            "speedup": (upd * steps / ms_batch) / (upd * (steps - 1) / ms_seq),
            "autotune_s": t_tune, "rows_per_workgroup_chosen": [int(h) for h in chosen],
+           "fp32_launch": "persistent queue" if batch.uses_queue() else "one launch per class",
            "bitwise_equal_to_sequential": bool(same)}
     batch.close()
     for c in A + B:

@@ -272,10 +272,17 @@ const char *rt_batch_last_error(const rt_batch *b);
 int rt_batch_step(rt_batch *b, int32_t nSteps);
 /* As rt_batch_step, timed with HIP events on the batch's stream. */
 int rt_batch_time_steps(rt_batch *b, int32_t nSteps, float *ms);
-/* Picks the rows per workgroup of each class by timing 2*trial_steps steps
- * per candidate (the members' states are restored afterwards); chosen
- * (optional, one entry per class) gets the choice, 0 = the default rule. */
+/* The FP32 members step in one persistent launch that takes (problem, tile)
+ * items from a queue, largest tiles first (the default), or in one launch per
+ * (nDir, streaming) class.  rt_batch_autotune times 2*trial_steps steps per
+ * candidate (the members' states are restored afterwards): rows per tile of
+ * the queue, and rows per workgroup of each class, and keeps the queue when
+ * it is faster than the classes' launches together; chosen (optional, one
+ * entry per class) gets the rows per tile or workgroup, 0 = the default rule.
+ * No counterpart in the reference (a batching layer of this library). */
 int rt_batch_autotune(rt_batch *b, int32_t trial_steps, int32_t *chosen);
+/* *queue = 1 when the FP32 members step in the queue launch. */
+int rt_batch_uses_queue(const rt_batch *b, int32_t *queue);
 int rt_batch_groups(const rt_batch *b, int32_t *ngroups);
 
 /* ---- timing helper ------------------------------------------------------- */
