@@ -103,6 +103,7 @@ def lib():
     L.rt_batch_autotune.argtypes = [vp, i32, vp]
     L.rt_batch_groups.argtypes = [vp, C.POINTER(i32)]
     L.rt_batch_uses_queue.argtypes = [vp, C.POINTER(i32)]
+    L.rt_batch_set_queue.argtypes = [vp, i32]
     L.rt_build_stencil.argtypes = [i32, _dp, _dp, _dp, _dp, vp, vp]
     L.rt_build_stencil_ghost.argtypes = [i32, i32, _dp, _dp, _dp, _dp, vp, vp]
     L.rt_set_rotation.argtypes = [vp, _dp, vp]
@@ -425,6 +426,10 @@ class Batch:
         q = C.c_int32()
         self._c(lib().rt_batch_uses_queue(self.h, C.byref(q)))
         return bool(q.value)
+
+    def set_queue(self, on):
+        """Step the FP32 members in one queue launch (True) or one launch per class."""
+        self._c(lib().rt_batch_set_queue(self.h, 1 if on else 0))
 
     def autotune(self, trial_steps=2):
         """Rows per workgroup chosen per class (0 = default rule)."""

@@ -1094,6 +1094,14 @@ int rt_batch_autotune(rt_batch *b, int32_t trial_steps, int32_t *chosen)
     return capture_graph(b);
 }
 
+int rt_batch_set_queue(rt_batch *b, int32_t on)
+{
+    if (!b) return RT_ERR_ARG;
+    b->queue = on != 0;
+    if (int r = batch_prepare(b)) return r;
+    return capture_graph(b);
+}
+
 int rt_batch_uses_queue(const rt_batch *b, int32_t *queue)
 {
     if (!b || !queue) return RT_ERR_ARG;

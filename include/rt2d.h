@@ -281,8 +281,10 @@ int rt_batch_time_steps(rt_batch *b, int32_t nSteps, float *ms);
  * entry per class) gets the rows per tile or workgroup, 0 = the default rule.
  * No counterpart in the reference (a batching layer of this library). */
 int rt_batch_autotune(rt_batch *b, int32_t trial_steps, int32_t *chosen);
-/* *queue = 1 when the FP32 members step in the queue launch. */
+/* *queue = 1 when the FP32 members step in the queue launch; set_queue
+ * chooses (1) or leaves (0) it. */
 int rt_batch_uses_queue(const rt_batch *b, int32_t *queue);
+int rt_batch_set_queue(rt_batch *b, int32_t on);
 int rt_batch_groups(const rt_batch *b, int32_t *ngroups);
 
 /* ---- timing helper ------------------------------------------------------- */

@@ -1,5 +1,7 @@
-"""Batched stepping of independent problems (grouped launches replayed from a
-HIP graph) against the same problems stepped one by one."""
+"""Batched stepping of independent problems (replayed from a HIP graph: the
+FP32 members in one persistent launch draining a (problem, tile) queue, or
+one grouped launch per class) against the same problems stepped one by
+one."""
 import numpy as np
 import pytest
 
@@ -33,12 +35,16 @@ def members(rt):
 
 
 @pytest.mark.gpu
-def test_batch_matches_individual_steps(rt):
-    print("[batch] building members", flush=True)
+@pytest.mark.parametrize("queue", [True, False])
+def test_batch_matches_individual_steps(rt, queue):
+    print(f"[batch] building members, queue={queue}", flush=True)
     A = members(rt)
     B = members(rt)
     b = rt.Batch(A)
     assert b.groups() == 7
+    assert b.uses_queue()  # the default
+    b.set_queue(queue)
+    assert b.uses_queue() == queue
     for n in (7, 4):  # odd then even: both parities of the graph
         b.step(n)
         for c in B:
@@ -89,10 +95,18 @@ def test_full_batched_workload_matches_sequential(rt):
         B.append(rt.workloads.make_batch_member(sp, rt.FP32_MIXED)[0])
     assert 64 in {sp["nDir"] for sp in specs}
     b = rt.Batch(A)
+    assert b.uses_queue()
     for n in (3, 4):
         b.step(n)
         for c in B:
             c.step(n, want_moments=False)
+    for k, (a, c) in enumerate(zip(A, B)):
+        assert np.array_equal(a.download_I(), c.download_I()), k
+    # and one launch per class for the next steps
+    b.set_queue(False)
+    b.step(3)
+    for c in B:
+        c.step(3, want_moments=False)
     for k, (a, c) in enumerate(zip(A, B)):
         assert np.array_equal(a.download_I(), c.download_I()), k
     b.close()
