@@ -982,10 +982,49 @@ static int check_adjoint_ready(rt_ctx *c)
     return RT_OK;
 }
 
+/* Segment storage of the adjoint: two segments of `every` states (tail
+ * guard, as d_I) and their records, and the second stream that recomputes
+ * one segment while the other is read. */
+static int ensure_segments(rt_ctx *c, int every)
+{
+    const size_t N = c->N, bytes = N * c->D * tsize(c);
+    if (every > c->seg_cap) {
+        c->seg_valid = false;
+        dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
+        c->seg_cap = 0;
+        TRY(dalloc(c, &c->d_seg, 2 * (size_t)every * bytes + 256));
+        TRY(dalloc(c, &c->d_rec_m, 2 * (size_t)every * 6 * N * sizeof(double)));
+        TRY(dalloc(c, &c->d_rec_k, 2 * (size_t)every * N));
+        TRY(dalloc(c, &c->d_rec_mask, 2 * (size_t)every * 8 * N * sizeof(unsigned long long)));
+        c->seg_cap = every;
+    }
+    if (!c->stream2) {
+        /* default priority, like `stream` */
+        RT_HIP(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        for (hipEvent_t *e : {&c->ev_seg[0], &c->ev_seg[1], &c->ev_adj[0], &c->ev_adj[1], &c->ev_go})
+            RT_HIP(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
+    }
+    return RT_OK;
+}
+static char *seg_state(rt_ctx *c, int b, int i)
+{
+    return (char *)c->d_seg + ((size_t)b * c->ck_every + i) * ((size_t)c->N * c->D * tsize(c));
+}
+static RecordPtrs seg_record(rt_ctx *c, int b, int i)
+{
+    const size_t k = (size_t)b * c->ck_every + i, N = c->N;
+    RecordPtrs r;
+    r.m = c->d_rec_m + k * 6 * N;
+    r.k = c->d_rec_k + k * N;
+    r.mask = c->d_rec_mask + k * 8 * N;
+    return r;
+}
+
 int rt_forward_checkpointed(rt_ctx *c, int32_t nSteps, int32_t every)
 {
     if (!c) return rt_fail(nullptr, RT_ERR_ARG, "null context");
     c->ck_valid = false;
+    c->seg_valid = false;
     if (nSteps < 0 || every < 1) return rt_fail(c, RT_ERR_ARG, "nSteps must be >= 0 and every >= 1");
     TRY(check_adjoint_ready(c));
     const size_t ND = (size_t)c->N * c->D, bytes = ND * tsize(c);
@@ -996,22 +1035,41 @@ int rt_forward_checkpointed(rt_ctx *c, int32_t nSteps, int32_t every)
         TRY(dalloc(c, &c->d_ckpt, (size_t)count * bytes + 256)); /* tail guard, as d_I */
         c->ck_cap = count;
     }
+    TRY(ensure_segments(c, every));
     TRY(prepare_fused(c));
     c->ck_scaled = c->scaled;
+    c->ck_every = every;
     for (int s = 0; s < count; s++) {
         RT_HIP(c, hipMemcpyAsync((char *)c->d_ckpt + (size_t)s * bytes, c->d_I[c->cur], bytes, hipMemcpyDeviceToDevice,
                                  c->stream));
         const int steps = std::min(every, nSteps - s * every);
-        for (int n = 0; n < steps; n++) {
-            TRY(launch_product_step(c, c->d_I[c->cur], c->d_I[1 - c->cur], false));
-            c->cur = 1 - c->cur;
+        if (s < count - 1) {
+            for (int n = 0; n < steps; n++) {
+                TRY(launch_product_step(c, c->d_I[c->cur], c->d_I[1 - c->cur], false));
+                c->cur = 1 - c->cur;
+            }
+            continue;
         }
+        /* the last segment runs with recording into segment buffer 0 (the
+         * same arithmetic), so that the adjoint starts on it without
+         * recomputing it; its final state comes back to d_I */
+        /* fixed cells of every state of buffer 0: the adjoint later
+         * recomputes full segments into it */
+        for (int i = 0; i < std::min(every, nSteps); i++)
+            TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], seg_state(c, 0, i)));
+        const void *prev = c->d_I[c->cur];
+        for (int i = 0; i < steps; i++) {
+            TRY(launch_fused_step_rec(c, prev, seg_state(c, 0, i), seg_record(c, 0, i)));
+            prev = seg_state(c, 0, i);
+        }
+        if (steps > 0)
+            RT_HIP(c, hipMemcpyAsync(c->d_I[c->cur], prev, bytes, hipMemcpyDeviceToDevice, c->stream));
     }
     RT_HIP(c, hipStreamSynchronize(c->stream));
     c->ck_steps = nSteps;
-    c->ck_every = every;
     c->ck_count = count;
     c->ck_valid = true;
+    c->seg_valid = count > 0;
     return RT_OK;
 }
 
@@ -1087,22 +1145,7 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
     if (!all_finite(dL_dout, (size_t)npts * c->D)) return rt_fail(c, RT_ERR_NONFINITE, "dL_dout contains NaN/Inf");
     const size_t N = c->N, ND = N * c->D, bytes = ND * tsize(c);
     const int every = c->ck_every;
-    if (every > c->seg_cap) {
-        dfree(c->d_seg); dfree(c->d_rec_m); dfree(c->d_rec_k); dfree(c->d_rec_mask);
-        c->seg_cap = 0;
-        /* two segments: states (tail guard, as d_I) and records */
-        TRY(dalloc(c, &c->d_seg, 2 * (size_t)every * bytes + 256));
-        TRY(dalloc(c, &c->d_rec_m, 2 * (size_t)every * 6 * N * sizeof(double)));
-        TRY(dalloc(c, &c->d_rec_k, 2 * (size_t)every * N));
-        TRY(dalloc(c, &c->d_rec_mask, 2 * (size_t)every * 8 * N * sizeof(unsigned long long)));
-        c->seg_cap = every;
-    }
-    if (!c->stream2) {
-        /* default priority, like `stream` */
-        RT_HIP(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
-        for (hipEvent_t *e : {&c->ev_seg[0], &c->ev_seg[1], &c->ev_adj[0], &c->ev_adj[1], &c->ev_go})
-            RT_HIP(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
-    }
+    TRY(ensure_segments(c, every));
     TRY(dalloc(c, &c->d_lam[0], bytes));
     TRY(dalloc(c, &c->d_lam[1], bytes));
     TRY(dalloc(c, &c->d_grad[0], N * sizeof(double)));
@@ -1125,18 +1168,14 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
     TRY(launch_sample_adjoint(c, c->d_lam[lc], c->ck_scaled ? 1 : 0, npts, c->d_pts, c->d_pts + npts,
                               c->d_pts + 2 * npts));
     /* segment buffer b (0, 1): the state after step i and its records */
-    auto seg_at = [&](int b, int i) { return (char *)c->d_seg + ((size_t)b * every + i) * bytes; };
-    auto rec_at = [&](int b, int i) {
-        const size_t k = (size_t)b * every + i;
-        RecordPtrs r;
-        r.m = c->d_rec_m + k * 6 * N;
-        r.k = c->d_rec_k + k * N;
-        r.mask = c->d_rec_mask + k * 8 * N;
-        return r;
-    };
+    auto seg_at = [&](int b, int i) { return seg_state(c, b, i); };
+    auto rec_at = [&](int b, int i) { return seg_record(c, b, i); };
+    /* buffer 0 already holds the last segment, recorded by the forward run */
+    const bool have_last = c->seg_valid;
+    c->seg_valid = false; /* the recomputation below overwrites it */
     /* halo and excised cells keep their values in every state: copy them
      * into the segment buffers once */
-    for (int b = 0; b < 2; b++)
+    for (int b = have_last ? 1 : 0; b < 2; b++)
         for (int i = 0; i < std::min(every, c->ck_steps); i++)
             TRY(launch_copy_fixed_cells(c, c->d_ckpt, seg_at(b, i)));
     /* the adjoint step clears the active cells of the buffer it reads, which
@@ -1163,7 +1202,7 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
     RT_HIP(c, hipStreamWaitEvent(c->stream2, c->ev_go, 0));
     const int last = c->ck_count - 1;
     if (last >= 0) {
-        TRY(recompute(last, 0));
+        if (!have_last) TRY(recompute(last, 0));
         RT_HIP(c, hipEventRecord(c->ev_seg[0], c->stream2));
     }
     for (int s = last, b = 0; s >= 0; s--, b ^= 1) {

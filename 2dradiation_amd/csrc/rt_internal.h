@@ -162,6 +162,9 @@ struct rt_ctx {
     /* the adjoint recomputes segment s-1 on stream2 while segment s's adjoint
      * steps run on `stream`: d_seg and the records hold two segments */
     hipStream_t stream2 = nullptr;
+    /* segment buffer 0 holds the last segment of the latest
+     * rt_forward_checkpointed run, recorded as it ran */
+    bool seg_valid = false;
     hipEvent_t ev_seg[2] = {nullptr, nullptr}, ev_adj[2] = {nullptr, nullptr}, ev_go = nullptr;
     int rec_waves = RT_FUSED_WAVES;             /* wavefronts of the last recording launch (4 or 8) */
     void *d_lam[2] = {nullptr, nullptr};

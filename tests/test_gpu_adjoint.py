@@ -181,3 +181,29 @@ def test_autograd_backward_after_another_forward_raises(rt, O):
     ge, _ = ctx.adjoint(px, py, out2.detach().numpy())
     assert np.allclose(eta.grad.numpy(), ge, rtol=0, atol=1e-14 * np.abs(ge).max())
     ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("steps,every", [(13, 4), (12, 4), (5, 8), (2, 1)])
+def test_adjoint_segment_layouts_and_repeat(rt, O, steps, every):
+    """The checkpointed forward records its last segment (shorter than,
+    equal to, or the only segment); the first adjoint starts on it, a second
+    adjoint after the same forward recomputes it.  Both against the FP64
+    adjoint oracle, and against each other."""
+    print(f"[adjoint] layout steps={steps} every={every}", flush=True)
+    P, ctx, g, fields, I0 = build(rt, O, 1, 40, 4.0, 16, rt.FP64)
+    px, py = ring(32, 3.0)
+    states = P.adj_forward(I0, steps)
+    _, _, lamT = P.observer_loss(states[-1], px, py, np.zeros((32, 16)), want_grad=True)
+    ge_ref, gk_ref = P.adjoint(states, lamT)
+    ctx.upload_I(I0)
+    ctx.forward_checkpointed(steps, every)
+    obs = ctx.sample_outgoing(px, py)
+    assert rel_l2(obs, P.observer_loss(states[-1], px, py, np.zeros((32, 16)))[1]) <= 1e-12
+    ge1, gk1 = ctx.adjoint(px, py, obs)
+    ge2, gk2 = ctx.adjoint(px, py, obs)
+    for a, b, ref in ((ge1, ge2, ge_ref), (gk1, gk2, gk_ref)):
+        assert rel_l2(a, ref) <= 1e-9
+        assert rel_l2(b, ref) <= 1e-9
+        assert rel_l2(a, b) <= 1e-12
+    ctx.close()
