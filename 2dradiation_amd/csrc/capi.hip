@@ -189,13 +189,18 @@ int build_dir_tables(rt_ctx *c, const double *w, const double *phi, const double
          * would act as one coherent gain on all rays */
         exact_sum_f32(h->wA[r], 1.0, h->wAf[r]);
         exact_sum_f32(h->wD[r], 0.0, h->wDf[r]);
+        /* A - D, exact (both on the 2^-23 grid, difference below 2): the
+         * record-driven FP32 stream forms C = (A - D) + (1 + t) D with the
+         * fraction t as 1 + t (geometry_from_record) */
         for (int m = 0; m < 5; m++) {
-            h->wrowf[r][m] = h->wAf[r][m];
+            h->wrowf[r][m] = (float)((double)h->wAf[r][m] - (double)h->wDf[r][m]);
             h->wrowf[r][5 + m] = h->wDf[r][m];
         }
         std::memcpy(&h->wrowf[r][10], &h->woff[r], sizeof(float));
         h->wrowf[r][11] = 0.0f;
     }
+    /* row 10: all weights 0, the window of an excised departure (k_angle_rec) */
+    for (int m = 0; m < 12; m++) h->wrowf[10][m] = 0.0f;
     c->fused_tables_ok = ok;
     hipError_t e = hipMemcpyAsync(c->d_dirt, h, sizeof *h, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

@@ -123,13 +123,14 @@ __device__ inline void geometry_from_series(T s, T di, T dj, T vx, T vy, CDirTab
     asm("v_mad_i32_i24 %0, %1, -10, %2" : "=v"(r) : "v"(q10), "v"(g0));
     int woff;
     if constexpr (sizeof(T) == 4) {
+        /* the rows hold A - D and D (A = (A - D) + D exactly) */
         const float4 *wr = (const float4 *)((const char *)wrow + __umul24((unsigned)r, 48u));
         const float4 w0 = wr[0], w1 = wr[1], w2 = wr[2];
-        G.Cw[0] = w0.x + t * w1.y;
-        G.Cw[1] = w0.y + t * w1.z;
-        G.Cw[2] = w0.z + t * w1.w;
-        G.Cw[3] = w0.w + t * w2.x;
-        G.Cw[4] = w1.x + t * w2.y;
+        G.Cw[0] = (w0.x + w1.y) + t * w1.y;
+        G.Cw[1] = (w0.y + w1.z) + t * w1.z;
+        G.Cw[2] = (w0.z + w1.w) + t * w1.w;
+        G.Cw[3] = (w0.w + w2.x) + t * w2.x;
+        G.Cw[4] = (w1.x + w2.y) + t * w2.y;
         woff = __float_as_int(w2.z);
     } else {
         /* FP64 rows come through the vector L1 (faster than LDS for 8-byte
@@ -161,40 +162,45 @@ __device__ inline void angle_window64(double vx, double vy, CDirTables *dt, int 
     s0 = q10 + dt->woff[r] + RT_RING_PAD;
 }
 
-/* Angular record of one (cell, direction): s0 in bits 25..31, r in 21..24,
- * t in 21 fixed-point bits (rounded, at most 1 - 2^-21). */
+/* Angular record of one (cell, direction): s0 in bits 23..29, t in 19
+ * fixed-point bits 4..22 (rounded, at most 1 - 2^-19), r in bits 0..3.  The
+ * bits of t sit where an FP32 mantissa keeps them, so 1 + t is one bitwise
+ * operation.  r = 10 (all weights 0) marks a departure point inside the
+ * excision radius, which gathers nothing. */
 __device__ inline uint32_t pack_angle_rec(int s0, int r, double t)
 {
-    const double tq = fmin(rint(t * 2097152.0), 2097151.0);
-    return ((uint32_t)s0 << 25) | ((uint32_t)r << 21) | (uint32_t)tq;
+    const double tq = fmin(rint(t * 524288.0), 524287.0);
+    return ((uint32_t)s0 << 23) | ((uint32_t)tq << 4) | (uint32_t)r;
 }
 
-/* FP32 geometry from the S, X and Y series values and the angular record
- * (see geometry_from_series for the departure part).  Records are valid by
- * construction (k_angle_rec, or zero before the first tables): s0 <= D + 1
- * keeps the taps in the ring and r <= 9 in the weight table. */
+/* FP32 geometry from the S, X and Y series values and the angular record.
+ * Records are valid by construction (k_angle_rec, or zero before the first
+ * tables): s0 <= D + 1 keeps the taps in the ring and r <= 10 in the weight
+ * table.  The departure offsets (di, dj) lie in (-1, 1) (the fused path
+ * requires a departure bound below 0.999 cells), so the corner is the
+ * cell's own or the one before it, and v_fract_f32 gives the bilinear weight
+ * di - floor(di) in one step.  C = (A - D) + (1 + t) D is the FMA of
+ * A + t D: the same exact value, rounded once. */
 template <int D>
 __device__ inline void geometry_from_record(float s, float di, float dj, uint32_t rec, const float *wrow,
                                             DirGeom<float> &G)
 {
     G.s = s;
-    const float fi = fminf(fmaxf(floorf(di), -1.0f), 0.0f);
-    const float fj = fminf(fmaxf(floorf(dj), -1.0f), 0.0f);
-    G.fx = di - fi;
-    G.fy = dj - fj;
-    G.lo_i = fi < 0.0f;
-    G.lo_j = fj < 0.0f;
-    const int r = (int)((rec >> 21) & 15u);
-    const float t = (float)(rec & 0x1FFFFFu) * 0x1p-21f;
-    const float4 *wr = (const float4 *)((const char *)wrow + __umul24((unsigned)r, 48u));
+    G.fx = __builtin_amdgcn_fractf(di);
+    G.fy = __builtin_amdgcn_fractf(dj);
+    G.lo_i = di < 0.0f;
+    G.lo_j = dj < 0.0f;
+    const unsigned r = rec & 15u;
+    const float t1 = __uint_as_float((rec & 0x007FFFF0u) | 0x3F800000u);
+    const float4 *wr = (const float4 *)((const char *)wrow + __umul24(r, 48u));
     const float4 w0 = wr[0], w1 = wr[1];
     const float2 w2 = *(const float2 *)(wr + 2);
-    G.Cw[0] = w0.x + t * w1.y;
-    G.Cw[1] = w0.y + t * w1.z;
-    G.Cw[2] = w0.z + t * w1.w;
-    G.Cw[3] = w0.w + t * w2.x;
-    G.Cw[4] = w1.x + t * w2.y;
-    G.s0 = (int)(rec >> 25);
+    G.Cw[0] = fmaf(t1, w1.y, w0.x);
+    G.Cw[1] = fmaf(t1, w1.z, w0.y);
+    G.Cw[2] = fmaf(t1, w1.w, w0.z);
+    G.Cw[3] = fmaf(t1, w2.x, w0.w);
+    G.Cw[4] = fmaf(t1, w2.y, w1.x);
+    G.s0 = (int)(rec >> 23);
 }
 
 template <typename T, int D>

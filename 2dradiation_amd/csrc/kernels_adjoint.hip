@@ -60,7 +60,7 @@ struct ASmem {
     T C[CURVED ? 25 : 1][64];               /* coefficient series (d_coefT form) */
     unsigned long long EX[CURVED ? 64 : 1]; /* excised-departure bits */
     double X[RT_FUSED_WAVES][4][64];
-    float wrow[10][12] __attribute__((aligned(16)));
+    float wrow[11][12] __attribute__((aligned(16)));
 };
 
 template <typename T, int D, bool CURVED>
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
     float bas4[(CURVED && sizeof(T) == 4) ? NG : 1][4];
     if constexpr (CURVED && sizeof(T) == 4) basis_groups<DW, NG>(dtab0, wv, lane, bas4);
     if constexpr (CURVED && sizeof(T) == 4) {
-        if (threadIdx.x < 120) (&sm.wrow[0][0])[threadIdx.x] = (&dtab0->wrowf[0][0])[threadIdx.x];
+        if (threadIdx.x < 132) (&sm.wrow[0][0])[threadIdx.x] = (&dtab0->wrowf[0][0])[threadIdx.x];
     }
     auto stage_coef = [&](int row) {
         if constexpr (CURVED) {

@@ -46,7 +46,7 @@ struct DirTables {
     double wA[10][5], wD[10][5];   /* combined 5-tap angular weights: C = A + t*D */
     float wAf[10][5], wDf[10][5];
     /* FP32 row per residue for 16-byte LDS reads: wA[5], wD[5], woff (int bits), 0 */
-    float wrowf[10][12];
+    float wrowf[11][12];
     double ang_u;        /* nGrid / (2 pi) */
     float ang_uf;
     int quarter;         /* nGrid / 4 */
