@@ -444,8 +444,9 @@ KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, con
      * checkpoints and segment states of the adjoint) carries a 256-byte tail
      * guard, or is followed by another state of the same allocation */
     auto a16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
+    /* (the per-lane DMA offsets of a row are 32-bit: D N sizeof(T) < 2^32) */
     p.wide = (sizeof(T) == 4 && d.nx % 2 == 0 && p.N % 4 == 0 && a16(in) && a16(c->d_cell) && a16(c->d_coefT) &&
-              a16(c->d_excl))
+              a16(c->d_excl) && (double)c->D * (double)p.N * sizeof(T) < 4294967296.0)
                  ? 1
                  : 0;
     return p;
