@@ -54,8 +54,9 @@ struct AP {
 template <typename T, int D, bool CURVED>
 struct ASmem {
     static constexpr int RW = 66;
-    /* adjoint accumulation, slot = row % 3; FP64 in both precisions (LDS
-     * FP32 atomic adds measured about 4x slower here than FP64 ones) */
+    /* adjoint accumulation, slot = row % 3; FP64 in both precisions
+     * (ds_add_f64: the FP32 LDS add, ds_add_f32, measured about 5x slower
+     * here) */
     double A[3][RT_RING_ROWS(D)][RW];
     T C[CURVED ? 25 : 1][64];               /* coefficient series (d_coefT form) */
     unsigned long long EX[CURVED ? 64 : 1]; /* excised-departure bits */
@@ -324,20 +325,22 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
                         const T s3 = T(1) + G.s * (T(3) + G.s * (T(3) + G.s));
                         const T f = (T)jb * s3;
                         const unsigned bits = (unsigned)(mbits >> (4 * k)) & 15u;
-                        const T w00 = (bits & 1u) ? f * (T(1) - G.fx) * (T(1) - G.fy) : T(0);
-                        const T w01 = (bits & 2u) ? f * (T(1) - G.fx) * G.fy : T(0);
-                        const T w10 = (bits & 4u) ? f * G.fx * (T(1) - G.fy) : T(0);
-                        const T w11 = (bits & 8u) ? f * G.fx * G.fy : T(0);
+                        /* corner weights to FP64 once; the 20 products in FP64 */
+                        const double w00 = (bits & 1u) ? (double)(f * (T(1) - G.fx) * (T(1) - G.fy)) : 0.0;
+                        const double w01 = (bits & 2u) ? (double)(f * (T(1) - G.fx) * G.fy) : 0.0;
+                        const double w10 = (bits & 4u) ? (double)(f * G.fx * (T(1) - G.fy)) : 0.0;
+                        const double w11 = (bits & 8u) ? (double)(f * G.fx * G.fy) : 0.0;
                         const int ra = G.lo_j ? slm : sl0, rb = G.lo_j ? sl0 : slp;
                         const int ca = lane + (G.lo_i ? 0 : 1);
 #pragma unroll
                         for (int m = 0; m < 5; m++) {
                             double *pa = &sm.A[ra][G.s0 + m][ca];
                             double *pb = &sm.A[rb][G.s0 + m][ca];
-                            atomicAdd(pa, (double)(w00 * G.Cw[m]));
-                            atomicAdd(pa + 1, (double)(w10 * G.Cw[m]));
-                            atomicAdd(pb, (double)(w01 * G.Cw[m]));
-                            atomicAdd(pb + 1, (double)(w11 * G.Cw[m]));
+                            const double cw = (double)G.Cw[m];
+                            atomicAdd(pa, w00 * cw);
+                            atomicAdd(pa + 1, w10 * cw);
+                            atomicAdd(pb, w01 * cw);
+                            atomicAdd(pb + 1, w11 * cw);
                         }
                     }
                 }

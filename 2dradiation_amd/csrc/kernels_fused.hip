@@ -257,7 +257,7 @@ __device__ inline void collide_moving(T (&sv)[DW], double (&mOut)[6], int &cycle
 
 /* Wavefronts per SIMD the register allocation must allow: as many
  * workgroups per CU as their LDS allows in 160 KB (at most 3), each with
- * NW/4 wavefronts on every SIMD, at most 4 in all. */
+ * NW/4 wavefronts on every SIMD, at most 2 (FP32) or 4 (FP64) in all. */
 template <typename T, int D, bool CURVED, int NW>
 constexpr int fused_waves_per_simd()
 {
@@ -265,7 +265,11 @@ constexpr int fused_waves_per_simd()
     constexpr size_t lds = (sizeof(Smem<T, D, CURVED, NW>) + 511) / 512 * 512;
     constexpr int by_lds = (int)((160 * 1024) / lds);
     constexpr int wgs = by_lds < 1 ? 1 : (by_lds > 3 ? 3 : by_lds);
-    return wgs * NW / 4 > 4 ? 4 : wgs * NW / 4;
+    /* FP32 forms stop at 2: with 3 per SIMD their 168 registers spill
+     * (256^2 x 32 Schwarzschild: 16.5 us per step at 2 against 19.8 us at
+     * 3) */
+    constexpr int cap = sizeof(T) == 4 ? 2 : 4;
+    return wgs * NW / 4 > cap ? cap : wgs * NW / 4;
 }
 
 /* XCD-aware tile order: consecutive workgroups are dealt round-robin over
