@@ -153,22 +153,70 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
             }
         }
     };
+    /* FP32-mixed: 1/x as v_rcp_f64 refined by one Newton step (x >= 1
+     * here); the FP64 kernel divides */
+    auto inv = [](double x) -> double {
+        if constexpr (sizeof(T) == 4) {
+            const double r = __builtin_amdgcn_rcp(x);
+            return fma(r, fma(-x, r, 1.0), r);
+        } else {
+            return 1.0 / x;
+        }
+    };
+    /* Per-cell inputs of a row, loaded one row ahead (issued after the
+     * previous row's scatter, so their latency passes behind its flush and
+     * barriers): the adjoint after the step (cleared in lam_in as read) and
+     * the state after it for this wavefront's directions, the mask, fields,
+     * recorded moments, cycle count, clamp bits and angular records. */
+    constexpr bool AREC = CURVED && sizeof(T) == 4;
+    T r_lam[DW], r_In[DW];
+    uint32_t arec[AREC ? DW : 1];
+    T r_f[6];
+    double r_m[3];
+    int r_K = 0;
+    uint8_t r_mk = 0;
+    unsigned long long r_mlo = 0, r_mhi = 0;
+    auto load_row = [&](int row) {
+        const unsigned ijr = (unsigned)row * (unsigned)nx + (unsigned)gi;
+        r_mk = inside ? p.mask[ijr] : 0;
+        const bool act = (r_mk & MASK_ACTIVE) != 0;
+#pragma unroll
+        for (int k = 0; k < DW; k++) {
+            const int d = wv * DW + k;
+            r_lam[k] = act ? (p.lam_in + (size_t)d * N)[ijr] : T(0);
+            if (act) (p.lam_in + (size_t)d * N)[ijr] = T(0);
+            r_In[k] = act ? (p.Inext + (size_t)d * N)[ijr] : T(0);
+        }
+        if constexpr (AREC) {
+#pragma unroll
+            for (int k = 0; k < DW; k++) arec[k] = inside ? (p.arec + (size_t)(wv * DW + k) * N)[ijr] : 0u;
+        }
+        constexpr int FIDX[6] = {CF_ETA, CF_K0, CF_K1, CF_KA, CF_ADT, CF_IA3};
+#pragma unroll
+        for (int f = 0; f < 6; f++) r_f[f] = act ? (p.cell + (size_t)FIDX[f] * N)[ijr] : T(f == 5 ? 1 : 0);
+#pragma unroll
+        for (int X = 0; X < 3; X++) r_m[X] = act ? (p.rec_m + (size_t)X * N)[ijr] : 0.0;
+        r_K = act ? p.rec_k[ijr] : 0;
+        if constexpr (CURVED) {
+            r_mlo = r_mhi = 0ull;
+            if (act) {
+                const int w0 = p.rec_waves == 8 ? 2 * wv : wv;
+                r_mlo = D <= 32 ? ((const uint32_t *)p.rec_mask + (size_t)w0 * N)[ijr] : (p.rec_mask + (size_t)w0 * N)[ijr];
+                if (p.rec_waves == 8)
+                    r_mhi = D <= 32 ? ((const uint32_t *)p.rec_mask + (size_t)(w0 + 1) * N)[ijr]
+                                    : (p.rec_mask + (size_t)(w0 + 1) * N)[ijr];
+            }
+        }
+    };
     stage_coef(rowA);
+    load_row(rowA);
 
     for (int j = rowA; j < rowB; j++) {
         __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads(); /* coefficients of row j staged; accumulation slot of row j+1 clear */
+        __syncthreads(); /* coefficients and per-cell inputs of row j in place; accumulation slot of row j+1 clear */
         const bool more = (j + 1 < rowB);
         const unsigned ij = (unsigned)j * (unsigned)nx + (unsigned)gi;
-        /* FP32 curved: the forward's angular windows of this row, read
-         * first (used by the scatter at the end of the row) */
-        constexpr bool AREC = CURVED && sizeof(T) == 4;
-        uint32_t arec[AREC ? DW : 1];
-        if constexpr (AREC) {
-#pragma unroll
-            for (int k = 0; k < DW; k++) arec[k] = inside ? (p.arec + (size_t)(wv * DW + k) * N)[ij] : 0u;
-        }
-        const uint8_t mk = inside ? p.mask[ij] : 0;
+        const uint8_t mk = r_mk;
         const bool active = (mk & MASK_ACTIVE) != 0;
         CDirTables *dtab = dtab0;
         asm volatile("" : "+s"(dtab));
@@ -179,9 +227,8 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
 #pragma unroll
         for (int k = 0; k < DW; k++) {
             const int d = wv * DW + k;
-            lam[k] = active ? (p.lam_in + (size_t)d * N)[ij] : T(0);
-            if (active) (p.lam_in + (size_t)d * N)[ij] = T(0);
-            const T In = active ? (p.Inext + (size_t)d * N)[ij] : T(0);
+            lam[k] = r_lam[k];
+            const T In = r_In[k];
             const double l = (double)lam[k];
             P0 += l;
             Px += l * dtab->cx[d];
@@ -201,22 +248,23 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
         double gb[3] = {0, 0, 0};
         double a3 = 1.0, ia3 = 1.0;
         if (active) {
-            const double eta = (double)(p.cell + (size_t)CF_ETA * N)[ij];
-            const double k0 = (double)(p.cell + (size_t)CF_K0 * N)[ij];
-            const double k1 = (double)(p.cell + (size_t)CF_K1 * N)[ij];
-            const double kA = (double)(p.cell + (size_t)CF_KA * N)[ij];
-            const double a = (double)(p.cell + (size_t)CF_ADT * N)[ij];
+            const double eta = (double)r_f[0];
+            const double k0 = (double)r_f[1];
+            const double k1 = (double)r_f[2];
+            const double kA = (double)r_f[3];
+            const double a = (double)r_f[4];
             if (CURVED) {
-                ia3 = (double)(p.cell + (size_t)CF_IA3 * N)[ij];
-                a3 = 1.0 / ia3;
+                ia3 = (double)r_f[5];
+                a3 = inv(ia3);
             }
-            const double q = 1.0 / (1.0 + a * (kA + k0));
+            const double den = 1.0 + a * (kA + k0);
+            const double q = inv(den);
             const double cc = 3.0 * k1 * a * q;
             const double dq = -a * q * q, dcc = 3.0 * k1 * a * dq;
             double mI[3];
 #pragma unroll
-            for (int X = 0; X < 3; X++) mI[X] = (p.rec_m + (size_t)X * N)[ij];
-            const int K = p.rec_k[ij];
+            for (int X = 0; X < 3; X++) mI[X] = r_m[X];
+            const int K = r_K;
             /* moments entering the last cycle and their tangents */
             double v[3] = {mI[0], mI[1], mI[2]}, te[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
             for (int cyc = 1; cyc < K; cyc++) {
@@ -241,7 +289,9 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
             /* adjoint sums in true intensity units */
             const double L0 = ia3 * Ls[0], Lx = ia3 * Ls[1], Ly = ia3 * Ls[2];
             const double b = a * q * (eta + k0 * v[0]);
-            const double LJ = (Ls[3] - b * L0 - cc * (v[1] * Lx + v[2] * Ly)) / q;
+            /* / q (the FP32-mixed kernel multiplies by den = 1/q) */
+            const double num = Ls[3] - b * L0 - cc * (v[1] * Lx + v[2] * Ly);
+            const double LJ = sizeof(T) == 4 ? num * den : num / q;
             if (wv == 0) {
                 p.grad_eta[ij] += a * q * (1.0 + k0 * te[0]) * L0 + cc * (te[1] * Lx + te[2] * Ly);
                 p.grad_k0[ij] += dq * LJ + a * (dq * (eta + k0 * v[0]) + q * v[0] + q * k0 * tk[0]) * L0 +
@@ -281,20 +331,10 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
                 cV[k] = AREC ? T(0) : sm.C[20 + k][lane];
             }
             const uint32_t exw = (uint32_t)(sm.EX[lane] >> (wv * DW));
-            unsigned long long mbits = 0ull;
-            if (active) {
-                if (p.rec_waves == 8) {
-                    /* recording wavefronts 2 wv and 2 wv + 1 hold this
-                     * wavefront's first and second DW/2 directions */
-                    const unsigned long long lo = D <= 32 ? ((const uint32_t *)p.rec_mask + (size_t)(2 * wv) * N)[ij]
-                                                          : (p.rec_mask + (size_t)(2 * wv) * N)[ij];
-                    const unsigned long long hi = D <= 32 ? ((const uint32_t *)p.rec_mask + (size_t)(2 * wv + 1) * N)[ij]
-                                                          : (p.rec_mask + (size_t)(2 * wv + 1) * N)[ij];
-                    mbits = lo | (hi << (2 * DW));
-                } else {
-                    mbits = D <= 32 ? ((const uint32_t *)p.rec_mask + (size_t)wv * N)[ij] : (p.rec_mask + (size_t)wv * N)[ij];
-                }
-            }
+            /* the recording wavefronts 2 wv and 2 wv + 1 of an 8-wavefront
+             * recording hold this wavefront's first and second DW/2
+             * directions */
+            const unsigned long long mbits = p.rec_waves == 8 ? (r_mlo | (r_mhi << (2 * DW))) : r_mlo;
             const T angu = (sizeof(T) == 4) ? (T)dtab->ang_uf : (T)dtab->ang_u;
             const int quarter = dtab->quarter, nGrid = dtab->nGrid;
             const T ddmax = T(nGrid) - T(1e-8);
@@ -365,6 +405,7 @@ __global__ __launch_bounds__(256, (adj_min_blocks<T, D, CURVED>())) void k_adjoi
                 }
             }
         }
+        if (more) load_row(j + 1);
         __syncthreads(); /* every scatter of row j is in the ring; row j-1 is complete */
         flush(j - 1);
         if (more) stage_coef(j + 1);
