@@ -260,10 +260,12 @@ int rt_adjoint(rt_ctx *ctx, int32_t npts, const double *x, const double *y, cons
 /* ---- batches of independent problems ----------------------------------------
  * No counterpart in the reference (one problem per process there).  Many
  * contexts, each fully set up (uneven grids, nDir, spacetimes allowed), are
- * advanced together: per step one grouped launch per (precision, nDir,
- * streaming) class, two steps captured in a HIP graph and replayed.  Members
- * must be covered by the fused step (rt_uses_fused_path == 1) and live on one
- * device; while a batch exists, each member can still be used on its own. */
+ * advanced together: per step the FP32 members in one persistent launch that
+ * drains a (problem, tile) queue (or one launch per (nDir, streaming) class,
+ * rt_batch_set_queue), the FP64 members as rt_step steps them, two steps
+ * captured in a HIP graph and replayed.  Members must be covered by the
+ * fused step (rt_uses_fused_path == 1) and live on one device; while a batch
+ * exists, each member can still be used on its own. */
 typedef struct rt_batch rt_batch;
 int rt_batch_create(rt_ctx *const *ctxs, int32_t n, rt_batch **out);
 int rt_batch_destroy(rt_batch *b);
