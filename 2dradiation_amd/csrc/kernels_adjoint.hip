@@ -501,12 +501,12 @@ int launch_adj(rt_ctx *c, const void *lam_in, void *lam_out, const void *Inext, 
     const int wgs = per_cu * c->ncu;
     int chunks = (wgs + strips - 1) / strips;
     int H = (rows + chunks - 1) / chunks;
-    /* chunks down to 3 rows: on small grids that fills more of the second
-     * resident workgroup per CU the LDS allows, at the price of more
-     * halo-row atomics (every element of a chunk's H + 2 completed rows takes
-     * one global atomic; FP32 256^2 x 32, 500 steps with their recomputation:
-     * 2 rows 32.4 ms, 3 rows 30.9, 4 rows 36.9) */
-    H = H < 3 ? 3 : (H > 64 ? 64 : H);
+    /* chunks down to 2 rows: on small grids that fills the second resident
+     * workgroup per CU the LDS allows, at the price of more halo-row atomics
+     * (every element of a chunk's H + 2 completed rows takes one global
+     * atomic; FP32 256^2 x 32, 500 steps with their recomputation: 1 row
+     * 27.2 ms, 2 rows 22.7, 3 rows 24.8, 4 rows 26.6) */
+    H = H < 2 ? 2 : (H > 64 ? 64 : H);
     chunks = (rows + H - 1) / H;
     p.chunks = chunks;
     p.rows_per_block = H;

@@ -348,6 +348,7 @@ def main():
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-streaming leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other single-problem configs")
     ap.add_argument("--only-adaptive", action="store_true", help="run only the adaptive-streaming leg")
+    ap.add_argument("--only-adjoint", action="store_true", help="run only the adjoint leg")
     ap.add_argument("--ref-steps", type=int, default=2000,
                     help="steps from I = 0 of the comparison with the FP64 reference-order kernels "
                          "(default: the workload's full 2000; 0 skips it)")
@@ -370,6 +371,9 @@ def main():
     rt.lib()
     if args.only_adaptive:
         print(json.dumps({"adaptive": adaptive_leg(rt, cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.only_adjoint:
+        print(json.dumps({"adjoint": adjoint_leg(rt)}), flush=True)
         return
     spec = rt.workloads.WORKLOADS[WORKLOAD]
     K, W = args.steps, args.warmup
