@@ -106,6 +106,7 @@ int build_dir_tables(rt_ctx *c, const double *w, const double *phi, const double
 {
     const int D = c->D;
     c->fused_tables_ok = false;
+    c->z0_dirty = true;
     if (D > RT_MAX_FUSED_DIR) return RT_OK;
     DirTables *h = new DirTables();
     memset(h, 0, sizeof *h);
@@ -788,6 +789,10 @@ int prepare_fused(rt_ctx *c)
     if (!c->fixed_synced) {
         TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], c->d_I[1 - c->cur]));
         c->fixed_synced = true;
+    }
+    if (c->z0_dirty) {
+        TRY(launch_mark_zero_start(c));
+        c->z0_dirty = false;
     }
     return RT_OK;
 }

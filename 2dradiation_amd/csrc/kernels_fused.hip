@@ -70,10 +70,6 @@ struct KP {
     double *rec_m;
     uint8_t *rec_k;
     unsigned long long *rec_mask;
-    /* FP64 fields eta, kappa0, kappa1, kappaA, ux, uy [6][N] and lapse [N]
-     * (zero_start_stops) */
-    const double *f64;
-    const double *alpha64;
 };
 
 /* The reference's first lambda cycle from all-zero streamed intensities
@@ -105,6 +101,20 @@ __device__ __noinline__ bool zero_start_stops(const double *__restrict__ f64, co
         pyy += c * n2 * n2;
     }
     return e == 0.0 || fx == 0.0 || fy == 0.0 || pxx == 0.0 || pxy == 0.0 || pyy == 0.0;
+}
+
+/* zero_start_stops depends on a cell's fields and lapse only: evaluated once
+ * per field or table upload, it is kept as CF_ACT = 2 (active, first cycle
+ * from all-zero intensities stops), so the step reads a flag instead of
+ * running 6 D FP64 operations per such cell and row. */
+template <typename T, int D>
+__global__ void k_mark_zero_start(const double *__restrict__ f64, const double *__restrict__ alpha64,
+                                  const DirTables *dt, T *__restrict__ cell, long N, double dtime)
+{
+    const long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (ij >= N) return;
+    T *act = cell + (size_t)CF_ACT * N + ij;
+    if (*act != T(0) && zero_start_stops<D>(f64, alpha64, ij, N, dtime, (CDirTables *)dt)) *act = T(2);
 }
 
 /* LDS of one workgroup.  Intensity rows are SoA [direction][columns]
@@ -441,8 +451,6 @@ KP<T> make_kp(const rt_ctx *c, const void *in, void *out, bool want_moments, con
     p.rec_m = rec ? rec->m : nullptr;
     p.rec_k = rec ? rec->k : nullptr;
     p.rec_mask = rec ? rec->mask : nullptr;
-    p.f64 = c->d_f64;
-    p.alpha64 = c->d_alpha;
     /* 16-byte staging reads up to 2 columns past a row's last needed one:
      * every buffer the fused step reads a state from (d_I, and the
      * checkpoints and segment states of the adjoint) carries a 256-byte tail
@@ -558,6 +566,31 @@ int launch_product_step(rt_ctx *c, const void *in, void *out, bool want_moments)
 {
     if (!c->fp32) return launch_exact_step(c, in, out, want_moments, nullptr);
     return launch_fused_step(c, in, out, want_moments);
+}
+
+int launch_mark_zero_start(rt_ctx *c)
+{
+    const unsigned nb = (unsigned)((c->N + 255) / 256);
+    auto go = [&](auto tag, auto dtag) -> int {
+        using T = decltype(tag);
+        constexpr int D = decltype(dtag)::value;
+        k_mark_zero_start<T, D><<<nb, 256, 0, c->stream>>>(c->d_f64, c->d_alpha, c->d_dirt, (T *)c->d_cell, (long)c->N,
+                                                          c->desc.dt);
+        RT_HIP(c, hipGetLastError());
+        return RT_OK;
+    };
+    auto byD = [&](auto tag) -> int {
+        switch (c->D) {
+        case 16: return go(tag, std::integral_constant<int, 16>{});
+        case 20: return go(tag, std::integral_constant<int, 20>{});
+        case 24: return go(tag, std::integral_constant<int, 24>{});
+        case 32: return go(tag, std::integral_constant<int, 32>{});
+        case 48: return go(tag, std::integral_constant<int, 48>{});
+        case 64: return go(tag, std::integral_constant<int, 64>{});
+        default: return rt_fail(c, RT_ERR_UNSUPPORTED, "fused kernel: unsupported nDir");
+        }
+    };
+    return c->fp32 ? byD(float{}) : byD(double{});
 }
 
 int launch_fused_step_rec(rt_ctx *c, const void *in, void *out, const RecordPtrs &rec)

@@ -663,6 +663,7 @@ int launch_mask(rt_ctx *c)
 }
 int launch_prepare_cells(rt_ctx *c)
 {
+    c->z0_dirty = true;
     if (c->fp32)
         k_prepare<float><<<nblk(c->N, 256), 256, 0, c->stream>>>(c->d_f64, c->d_alpha, c->d_t11, c->d_t12, c->d_t21, c->d_t22,
                                                                  (float *)c->d_cell, c->d_mask, c->N, c->desc.dt);

@@ -113,6 +113,9 @@ struct rt_ctx {
     /* the fixed (halo and excised) cells of both state buffers are equal,
      * as the fused step needs; anything that writes them clears this */
     bool fixed_synced = false;
+    /* CF_ACT of d_cell lacks the zero-start marks (launch_mark_zero_start):
+     * set when the cell rows or the direction tables are rebuilt */
+    bool z0_dirty = true;
     double *d_mom = nullptr;      /* [6][N] */
     /* lab frame: tetrad [9][N] and gamma_ll [3][N]; LF moments [7][N] */
     double *d_frame = nullptr;
@@ -204,6 +207,7 @@ int launch_fused_step_rec(rt_ctx *c, const void *in, void *out, const RecordPtrs
  * for every cell with a finite positive lapse */
 int launch_scale_state(rt_ctx *c, void *buf, bool inverse);
 int launch_prepare_cells(rt_ctx *c);
+int launch_mark_zero_start(rt_ctx *c);
 int launch_ref_stream(rt_ctx *c, const void *in, void *out);
 /* adaptive streaming (kernels_adaptive.hip): rotation update (or only the
  * sines and cosines), the rotated stream, source-cell angles */

@@ -12,6 +12,14 @@ the timed steps, the HBM roofline of the fused kernel, the adjoint leg
 ratio, central-difference check), the batched leg (64 uneven problems in
 graph-replayed grouped launches vs one after another) and the CPU baseline
 (the oracle, OpenMP on the host cores, on a bounded number of steps).
+
+Before each timed leg of the headline workload the GPU is brought to its
+working clocks by CLOCK_SPINUP_S seconds of steps on a separate throwaway
+context (256x256 x 32): after the host-side set-up the first milliseconds of
+work otherwise run at lower clocks (steps 6..25 measured 0.306 ms/step
+without and 0.271 with it, against 0.257 for steps 201..220).  The timed
+region itself is unchanged: W warm-up and then exactly K steps of the
+workload from I = 0.
 """
 import argparse
 import importlib
@@ -27,6 +35,21 @@ sys.path.insert(0, ROOT)
 
 WORKLOAD = "schwarzschild_1024sq_d64"
 HBM_PEAK = 8.0e12
+CLOCK_SPINUP_S = 0.3
+_spinner = None
+
+
+def clock_spinup(rt, seconds=CLOCK_SPINUP_S):
+    """Steps a throwaway 256^2 x 32 context for `seconds` (untimed; no
+    state of the benchmarked contexts is touched), so the timed region that
+    follows starts at the GPU's working clocks."""
+    global _spinner
+    if _spinner is None:
+        _spinner, _ = rt.workloads.make_context("schwarzschild_256sq_d32", rt.FP32_MIXED)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        _spinner.step(100, want_moments=False)
+        _spinner.synchronize()
 
 
 def algorithmic_bytes_per_update(nDir, b_i, b_f, curved=True, nCoeff=5):
@@ -388,6 +411,7 @@ def main():
 
     # W untimed steps from I = 0, then K timed steps continuing from there
     # (no upload or state conversion between them)
+    clock_spinup(rt)
     c32.step(W, want_moments=False)
     c32.synchronize()
     c32.reset_iteration_stats()
@@ -409,7 +433,8 @@ def main():
         "data": "synthetic (seeded fields)",
         "config": {"workload": WORKLOAD, "grid": "1026x1026 stored (1024x1024 interior) on [-8,8]^2",
                    "nDir": spec["nDir"], "spacetime": "Schwarzschild M=1, excision r<=2.1",
-                   "cfl": spec["cfl"], "steps_per_run": K, "active_cells": act, "parallelism": "single GPU"},
+                   "cfl": spec["cfl"], "steps_per_run": K, "active_cells": act, "parallelism": "single GPU",
+                   "clock_spinup_s": CLOCK_SPINUP_S},
         "lambda_iterations": {"max": it_max, "mean": it_sum / max(1, act * K), "steps": f"{W + 1}..{W + K}"},
         "wall_s_fp32": wall32,
     }
@@ -428,6 +453,7 @@ def main():
     if not args.no_fp64:
         print(f"[bench] FP64 leg: {K} steps", file=sys.stderr, flush=True)
         c64, _ = rt.workloads.make_context(WORKLOAD, rt.FP64)
+        clock_spinup(rt)
         c64.step(W, want_moments=False)
         c64.synchronize()
         ms64 = c64.time_steps(K)
