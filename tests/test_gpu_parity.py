@@ -220,6 +220,52 @@ def test_curved_fp64_step_vs_reference_order_symmetric_start(rt, O, kind):
     assert a.iteration_stats() == b.iteration_stats()
 
 
+def _zero_start_stops_host(rt, gs, nDir, kind, f):
+    """Host evaluation of the reference's first-cycle 0/0 stop from all-zero
+    intensities (src/Radiation.cpp:661-721 with prev = 0 and u = 0): the
+    d-ordered sums of w v {1, n1, n2, n1 n1, n1 n2, n2 n2}, any exactly 0."""
+    w, phi, cx, cy = rt.build_stencil(nDir, tables=False)
+    _, alpha, *_ = rt.build_metric(kind, 1.0, 0.0, gs)
+    adt = alpha * gs.dt
+    den = 1.0 + alpha * gs.dt * (f.get("kappaA", 0.0) + f["kappa0"])
+    v = (0.0 + adt * f["eta"]) / den
+    sums = [np.zeros(gs.N) for _ in range(6)]
+    for d in range(nDir):
+        n1, n2 = cx[d], cy[d]
+        c = w[d] * v
+        for q, t in enumerate((c, c * n1, c * n2, c * n1 * n1, c * n1 * n2, c * n2 * n2)):
+            sums[q] = sums[q] + t
+    return np.logical_or.reduce([sq == 0.0 for sq in sums])
+
+
+def test_curved_fp32_zero_start_marks_follow_field_changes(rt, O):
+    """The first-cycle 0/0 stops from all-zero intensities are marked once per
+    field upload.  A context first stepped with eta = 0 (every active cell
+    marked) and then given other fields and I = 0 again must match, bit for
+    bit and in its cycle counts, a context built with those fields; the host
+    evaluation shows that the second fields mark some active cells and not
+    others, so stale marks would change the result."""
+    gs, _ = grids(rt, O, 96, 80, (-4, -4), (4, 4))
+    bh = 2.1 * 2.1
+    f = mixed_fields(rt, gs, 41, k0=1.0, k1=0.3, kA=0.0)
+    act = active_mask(gs, bh)
+    stops = _zero_start_stops_host(rt, gs, 32, 1, f)[act & (f["eta"] > 0)]
+    assert 0 < stops.sum() < stops.size, f"{stops.sum()} of {stops.size} cells stop"
+    f0 = dict(f)
+    f0["eta"] = np.zeros(gs.N)
+    a, _ = lib_context(rt, gs, 32, 1, f0, rt.FP32_MIXED, bh_r2=bh)
+    a.step(3)
+    a.set_fields(**f)
+    a.upload_I(np.zeros(gs.N * 32))
+    a.reset_iteration_stats()
+    a.step(6)
+    b, _ = lib_context(rt, gs, 32, 1, f, rt.FP32_MIXED, bh_r2=bh)
+    b.reset_iteration_stats()
+    b.step(6)
+    assert np.array_equal(a.download_I(), b.download_I())
+    assert a.iteration_stats() == b.iteration_stats()
+
+
 @pytest.mark.parametrize("kind", [1, 2])
 def test_curved_fp32_mixed_vs_fp64(rt, O, kind):
     gs, _ = grids(rt, O, 96, 80, (-4, -4), (4, 4))
