@@ -1046,6 +1046,12 @@ int rt_forward_checkpointed(rt_ctx *c, int32_t nSteps, int32_t every)
         c->ck_cap = count;
     }
     TRY(ensure_segments(c, every));
+    /* the adjoint's own buffers are allocated here, with the checkpoints,
+     * so that rt_adjoint allocates nothing */
+    TRY(dalloc(c, &c->d_lam[0], bytes));
+    TRY(dalloc(c, &c->d_lam[1], bytes));
+    TRY(dalloc(c, &c->d_grad[0], c->N * sizeof(double)));
+    TRY(dalloc(c, &c->d_grad[1], c->N * sizeof(double)));
     TRY(prepare_fused(c));
     c->ck_scaled = c->scaled;
     c->ck_every = every;
@@ -1063,10 +1069,12 @@ int rt_forward_checkpointed(rt_ctx *c, int32_t nSteps, int32_t every)
         /* the last segment runs with recording into segment buffer 0 (the
          * same arithmetic), so that the adjoint starts on it without
          * recomputing it; its final state comes back to d_I */
-        /* fixed cells of every state of buffer 0: the adjoint later
-         * recomputes full segments into it */
-        for (int i = 0; i < std::min(every, nSteps); i++)
-            TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], seg_state(c, 0, i)));
+        /* fixed cells (halo and excised, equal in every state of this run)
+         * of every state of both segment buffers: the recording steps and
+         * the adjoint's recomputation write active cells only */
+        for (int b = 0; b < 2; b++)
+            for (int i = 0; i < std::min(every, nSteps); i++)
+                TRY(launch_copy_fixed_cells(c, c->d_I[c->cur], seg_state(c, b, i)));
         const void *prev = c->d_I[c->cur];
         for (int i = 0; i < steps; i++) {
             TRY(launch_fused_step_rec(c, prev, seg_state(c, 0, i), seg_record(c, 0, i)));
@@ -1183,11 +1191,8 @@ int rt_adjoint(rt_ctx *c, int32_t npts, const double *x, const double *y, const 
     /* buffer 0 already holds the last segment, recorded by the forward run */
     const bool have_last = c->seg_valid;
     c->seg_valid = false; /* the recomputation below overwrites it */
-    /* halo and excised cells keep their values in every state: copy them
-     * into the segment buffers once */
-    for (int b = have_last ? 1 : 0; b < 2; b++)
-        for (int i = 0; i < std::min(every, c->ck_steps); i++)
-            TRY(launch_copy_fixed_cells(c, c->d_ckpt, seg_at(b, i)));
+    /* halo and excised cells of every segment state were filled by
+     * rt_forward_checkpointed (a valid checkpoint means an unchanged state) */
     /* the adjoint step clears the active cells of the buffer it reads, which
      * then takes the next step's scatter; what accumulates in fixed cells is
      * never read (fixed cells do not depend on active ones) */
