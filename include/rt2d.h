@@ -231,7 +231,9 @@ int rt_uses_fused_path(rt_ctx *ctx, int32_t *fused);
  * and absorption kappa0, for a loss on bilinear samples of the final state.
  * Requires rt_uses_fused_path == 1. */
 /* Runs nSteps steps like rt_step and keeps the state before steps 0, every,
- * 2*every, ... on the device for rt_adjoint (ceil(nSteps/every) states). */
+ * 2*every, ... on the device for rt_adjoint (ceil(nSteps/every) states), with
+ * the last segment recorded.  All device memory rt_adjoint uses (the two
+ * segment buffers, the adjoint state and the gradients) is allocated here. */
 int rt_forward_checkpointed(rt_ctx *ctx, int32_t nSteps, int32_t every);
 /* Bilinear samples of the current intensities at npts points (x, y): out
  * [npts*nDir] gets I_d at the point for the directions pointing outward
