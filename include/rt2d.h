@@ -232,8 +232,9 @@ int rt_uses_fused_path(rt_ctx *ctx, int32_t *fused);
  * Requires rt_uses_fused_path == 1. */
 /* Runs nSteps steps like rt_step and keeps the state before steps 0, every,
  * 2*every, ... on the device for rt_adjoint (ceil(nSteps/every) states), with
- * the last segment recorded.  All device memory rt_adjoint uses (the two
- * segment buffers, the adjoint state and the gradients) is allocated here. */
+ * the last segment recorded.  The segment buffers, the adjoint state and the
+ * gradients rt_adjoint uses are allocated here; rt_adjoint itself allocates
+ * only its point buffer, if no earlier rt_sample_outgoing sized it. */
 int rt_forward_checkpointed(rt_ctx *ctx, int32_t nSteps, int32_t every);
 /* Bilinear samples of the current intensities at npts points (x, y): out
  * [npts*nDir] gets I_d at the point for the directions pointing outward
