@@ -372,6 +372,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the other single-problem configs")
     ap.add_argument("--only-adaptive", action="store_true", help="run only the adaptive-streaming leg")
     ap.add_argument("--only-adjoint", action="store_true", help="run only the adjoint leg")
+    ap.add_argument("--only-batch", action="store_true", help="run only the batched-problems leg")
     ap.add_argument("--ref-steps", type=int, default=2000,
                     help="steps from I = 0 of the comparison with the FP64 reference-order kernels "
                          "(default: the workload's full 2000; 0 skips it)")
@@ -397,6 +398,9 @@ def main():
         return
     if args.only_adjoint:
         print(json.dumps({"adjoint": adjoint_leg(rt)}), flush=True)
+        return
+    if args.only_batch:
+        print(json.dumps({"batch": batch_leg(rt)}), flush=True)
         return
     spec = rt.workloads.WORKLOADS[WORKLOAD]
     K, W = args.steps, args.warmup
