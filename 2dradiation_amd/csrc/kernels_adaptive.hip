@@ -512,11 +512,11 @@ int launch_stream_adaptive(rt_ctx *c, const void *in, void *out)
  * fused kernel's moment-space iteration with per-cell stencil sums: the same
  * stopping rule on the moments, then one pass over the directions,
  *   Inew_d = I_d + ((q - 1) I_d + b + C(d).G),  G = c R(theta)^T F,
- * in the storage precision.  A workgroup is 64 cells x 4 wavefronts: wave 0
- * iterates (or, for a moving cell, runs the reference cycle,
- * ref_collide_cell, over all directions) and hands q - 1, b and G to the
- * waves through LDS; wave g then writes directions g, g+4, ... of the cells
- * at rest, 8 loads in flight at a time. */
+ * in the storage precision.  One lane is one cell: it iterates (or, for a
+ * moving cell, runs the reference cycle, ref_collide_cell, over all
+ * directions), then writes the directions of a cell at rest with 8 loads in
+ * flight at a time.  Every wavefront of a workgroup (256 consecutive cells)
+ * runs its own cells' iterations, none waits for another's. */
 struct StencilSums {
     double s[10];
 };
@@ -532,20 +532,16 @@ __global__ __launch_bounds__(256) void k_collide_adaptive(const T *__restrict__ 
                                                           int N, int D, double dt, unsigned long long *stats,
                                                           const double *__restrict__ rot, StencilSums ss)
 {
-    __shared__ T s_qm1[64], s_b[64], s_gx[64], s_gy[64];
-    __shared__ int s_rest[64];
     const int lane = threadIdx.x & 63;
-    const int g = threadIdx.x >> 6;
-    /* consecutive 64-cell chunks go to one XCD (workgroups are dealt
+    /* consecutive 256-cell chunks go to one XCD (workgroups are dealt
      * round-robin over the 8 XCDs): a direction row of 64 cells is not
      * 128-byte aligned, and its partial cache lines at both ends are then
      * written back by one L2 instead of two */
     const int nb = gridDim.x;
     const int chunk = (nb & 7) == 0 ? (int)(blockIdx.x & 7) * (nb >> 3) + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
-    const long ij = (long)chunk * 64 + lane;
-    if (g == 0) {
+    const long ij = (long)chunk * 256 + threadIdx.x;
+    {
         unsigned long long cyc = 0;
-        int rest = 0;
         if (ij < N) {
             if (!(mask[ij] & MASK_ACTIVE)) {
                 count[ij] = 0;
@@ -555,7 +551,6 @@ __global__ __launch_bounds__(256) void k_collide_adaptive(const T *__restrict__ 
                 const double u2 = t21[ij] * ux + t22[ij] * uy;
                 int cycles = 0;
                 if (u1 == 0.0 && u2 == 0.0) {
-                    rest = 1;
                     const double eta = f[ij], k0 = f[(long)N + ij], k1 = f[2L * N + ij], kA = f[3L * N + ij];
                     const double adt = alpha[ij] * dt;
                     const double q = 1.0 / (1.0 + adt * (kA + k0));
@@ -600,10 +595,22 @@ __global__ __launch_bounds__(256) void k_collide_adaptive(const T *__restrict__ 
                     }
 #pragma unroll
                     for (int X = 0; X < 6; X++) mom[(long)X * N + ij] = prev[X];
-                    s_qm1[lane] = (T)(q - 1.0);
-                    s_b[lane] = (T)b;
-                    s_gx[lane] = (T)(cc * (rc * fFx + rs * fFy));
-                    s_gy[lane] = (T)(cc * (rc * fFy - rs * fFx));
+                    const T qm1 = (T)(q - 1.0), bT = (T)b;
+                    const T gx = (T)(cc * (rc * fFx + rs * fFy)), gy = (T)(cc * (rc * fFy - rs * fFx));
+                    for (int d0 = 0; d0 < D; d0 += 8) {
+                        T v[8];
+#pragma unroll
+                        for (int k = 0; k < 8; k++)
+                            if (d0 + k < D) v[k] = I[(long)(d0 + k) * N + ij];
+#pragma unroll
+                        for (int k = 0; k < 8; k++) {
+                            const int d = d0 + k;
+                            if (d < D) {
+                                const T t = (T)cx[d] * gx + (T)cy[d] * gy;
+                                Inew[(long)d * N + ij] = v[k] + (qm1 * v[k] + (bT + t));
+                            }
+                        }
+                    }
                 } else {
                     cycles = ref_collide_cell<T>(ij, I, Inew, mom, f, alpha, t11, t12, t21, t22, w, cx, cy, N, D, dt, rot);
                 }
@@ -611,8 +618,7 @@ __global__ __launch_bounds__(256) void k_collide_adaptive(const T *__restrict__ 
                 cyc = (unsigned long long)cycles;
             }
         }
-        s_rest[lane] = rest;
-        /* cycle statistics: one atomic pair per workgroup */
+        /* cycle statistics: one atomic pair per wavefront */
         unsigned long long sum = cyc, mx = cyc;
         for (int off = 32; off > 0; off >>= 1) {
             sum += __shfl_xor(sum, off);
@@ -620,26 +626,9 @@ __global__ __launch_bounds__(256) void k_collide_adaptive(const T *__restrict__ 
             mx = mx > o ? mx : o;
         }
         if (lane == 0 && mx > 0) {
-            unsigned long long *st = rt_stat_slot(stats, blockIdx.x);
+            unsigned long long *st = rt_stat_slot(stats, blockIdx.x * 4u + (threadIdx.x >> 6));
             atomicAdd(&st[0], sum);
             atomicMax(&st[1], mx);
-        }
-    }
-    __syncthreads();
-    if (!s_rest[lane]) return;
-    const T qm1 = s_qm1[lane], bT = s_b[lane], gx = s_gx[lane], gy = s_gy[lane];
-    for (int d0 = g; d0 < D; d0 += 32) {
-        T v[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-            if (d0 + 4 * k < D) v[k] = I[(long)(d0 + 4 * k) * N + ij];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int d = d0 + 4 * k;
-            if (d < D) {
-                const T t = (T)cx[d] * gx + (T)cy[d] * gy;
-                Inew[(long)d * N + ij] = v[k] + (qm1 * v[k] + (bT + t));
-            }
         }
     }
 }
@@ -649,7 +638,7 @@ int launch_collide_adaptive(rt_ctx *c, const void *in, void *out)
     StencilSums ss;
     for (int q = 0; q < 10; q++) ss.s[q] = c->stencil_sums[q];
     const double *rot = c->d_rot[c->rcur];
-    const unsigned blocks = (nblk(c->N, 64) + 7) / 8 * 8;
+    const unsigned blocks = (nblk(c->N, 256) + 7) / 8 * 8;
     if (c->fp32)
         k_collide_adaptive<float><<<blocks, 256, 0, c->stream>>>(
             (const float *)in, (float *)out, c->d_mom, c->d_count, c->d_f64, c->d_alpha, c->d_t11, c->d_t12, c->d_t21,
