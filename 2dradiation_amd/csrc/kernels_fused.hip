@@ -300,6 +300,7 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks)
 template <typename T, int D, bool CURVED, bool REC, int NW, bool SONLY, typename PK>
 __device__ __forceinline__ void fused_tile(PK &p, int tile, const T *Iin, T *Iout, Smem<T, D, CURVED, NW> &sm)
 {
+    constexpr bool LEAN = true;
 #include "fused_tile_body.inc"
 }
 
@@ -313,6 +314,7 @@ __global__ __launch_bounds__(64 * NW, (fused_waves_per_simd<T, D, CURVED, NW>())
     const int tile = xcd_tile(blockIdx.x, p.nblocks);
     const T *Iin = p.Iin;
     T *Iout = p.Iout;
+    constexpr bool LEAN = false;
 #include "fused_tile_body.inc"
 }
 
@@ -334,6 +336,7 @@ __global__ __launch_bounds__(64 * RT_FUSED_WAVES, (fused_waves_per_simd<T, D, CU
     T *Iout = parity ? const_cast<T *>(p.Iin) : p.Iout;
     constexpr bool REC = false, SONLY = false;
     constexpr int NW = RT_FUSED_WAVES;
+    constexpr bool LEAN = false;
 #include "fused_tile_body.inc"
 }
 
